@@ -1,0 +1,147 @@
+"""Tensor -> C-ABI marshalling shared by the ``_C``, ``_C_benchmarking`` and
+``_C_stereo`` entry points.
+
+Mirrors the reference torch wrappers (rasterization/src/inference_api.cu:13-122,
+fast_inference_api.cu:12-121, inference_stereo_api.cu:13-192): the same
+positional arguments, the same output allocation (render: pixels no tile
+writes read 0; benchmark/stereo: left unwritten), work on the current stream.
+"""
+from __future__ import annotations
+
+import torch
+
+from . import _lib
+
+
+def _dev_ptr(t: torch.Tensor, name: str, dtypes, device) -> tuple[int, torch.Tensor]:
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch.Tensor")
+    if t.dtype not in dtypes:
+        raise TypeError(f"{name} must have dtype in {dtypes}, got {t.dtype}")
+    if t.device.type != "cuda":
+        raise ValueError(f"{name} must be a GPU tensor")
+    if device is not None and t.device != device:
+        raise ValueError(f"{name} is on {t.device}, expected {device}")
+    t = t.contiguous()
+    if t.data_ptr() % 16:
+        t = t.clone()  # vector loads need 16-byte aligned rows
+    return t.data_ptr(), t
+
+
+_F32 = (torch.float32,)
+_U32 = tuple(d for d in (getattr(torch, "uint32", None), torch.int32) if d is not None)
+
+
+def scene_struct(positions, scales, rotations, opacities, sh_0, sh_rest, keep: list) -> tuple:
+    device = positions.device
+    n = positions.shape[0]
+    if positions.dim() != 2 or positions.shape[1] != 3:
+        raise ValueError(f"positions must be (N,3), got {tuple(positions.shape)}")
+    if tuple(scales.shape) != (n, 3):
+        raise ValueError(f"scales must be (N,3), got {tuple(scales.shape)}")
+    if tuple(rotations.shape) != (n, 4):
+        raise ValueError(f"rotations must be (N,4), got {tuple(rotations.shape)}")
+    if opacities.numel() != n:
+        raise ValueError(f"opacities must have N elements, got {tuple(opacities.shape)}")
+    if sh_0.numel() != 3 * n:
+        raise ValueError(f"sh_0 must be (N,1,3), got {tuple(sh_0.shape)}")
+    if sh_rest.dim() != 3 or sh_rest.shape[0] != n or sh_rest.shape[2] != 3:
+        raise ValueError(f"sh_rest must be (N,S,3), got {tuple(sh_rest.shape)}")
+    s = _lib.Scene()
+    for field, t in (("positions", positions), ("scales", scales), ("rotations", rotations),
+                     ("opacities", opacities), ("sh_0", sh_0), ("sh_rest", sh_rest)):
+        p, tc = _dev_ptr(t, field, _F32, device)
+        keep.append(tc)
+        setattr(s, field, p)
+    s.n_primitives = n
+    s.total_sh_bases = int(sh_rest.shape[1])
+    return s, device
+
+
+def view_struct(device, M, VPM, VPR_inv, cam_position, gaze_position, visibility_mask,
+                visibility_mask_area_table, fovea_mask_area_table, background_model_data,
+                background_model_type, K, active_sh_bases, width, height, focal_x, focal_y,
+                center_x, center_y, near_plane, far_plane, scale_modifier, to_chw,
+                blur_periphery, anti_aliasing, foveation_radius_px, blend_width_px,
+                clear_output, keep: list) -> tuple:
+    v = _lib.View()
+    width, height = int(width), int(height)
+    if M.numel() < 12:
+        raise ValueError("M must hold at least 3 rows of 4")
+    if VPM.numel() != 16 or VPR_inv.numel() != 16:
+        raise ValueError("VPM and VPR_inv must be 4x4")
+    if cam_position.numel() != 3:
+        raise ValueError("cam_position must have 3 elements")
+    for field, t in (("M", M), ("VPM", VPM), ("VPR_inv", VPR_inv), ("cam_position", cam_position)):
+        p, tc = _dev_ptr(t, field, _F32, device)
+        keep.append(tc)
+        setattr(v, field, p)
+    # the reference reads the gaze through a host pointer (inference.cu:79-82)
+    g = gaze_position.detach().to("cpu", torch.float32).reshape(-1)
+    if g.numel() != 2:
+        raise ValueError("gaze_position must have 2 elements")
+    v.gaze_x, v.gaze_y = float(g[0]), float(g[1])
+    gw, gh = (width + 15) // 16, (height + 15) // 16
+    if visibility_mask.numel() < (gw * gh + 31) // 32:
+        raise ValueError("visibility_mask is smaller than ceil(tiles/32) words")
+    if visibility_mask_area_table.numel() < (gw + 1) * (gh + 1):
+        raise ValueError("visibility_mask_area_table is smaller than (gw+1)*(gh+1)")
+    R, _ = _lib.fovea_radius_tiles(width, height, int(foveation_radius_px), int(blend_width_px))
+    if fovea_mask_area_table.numel() < (2 * R + 1) ** 2:
+        raise ValueError(f"fovea_mask_area_table must hold (2*{R}+1)^2 entries")
+    for field, t in (("visibility_mask", visibility_mask),
+                     ("visibility_mask_area_table", visibility_mask_area_table),
+                     ("fovea_mask_area_table", fovea_mask_area_table)):
+        p, tc = _dev_ptr(t, field, _U32, device)
+        keep.append(tc)
+        setattr(v, field, p)
+    bg_type = int(background_model_type)
+    p, tc = _dev_ptr(background_model_data, "background_model_data", _F32, device)
+    keep.append(tc)
+    v.background_model_data = p
+    v.background_model_type = bg_type
+    if bg_type in (2, 3):
+        if background_model_data.dim() != 3 or background_model_data.shape[0] != 3:
+            raise ValueError("texture background must be (3,h,w)")
+        v.background_width = int(background_model_data.shape[2])
+        v.background_height = int(background_model_data.shape[1])
+    elif bg_type == 1 and background_model_data.numel() < 48:
+        raise ValueError("SH background needs 16x3 coefficients")
+    v.K = int(K)
+    v.active_sh_bases = int(active_sh_bases)
+    v.width, v.height = width, height
+    v.focal_x, v.focal_y = float(focal_x), float(focal_y)
+    v.center_x, v.center_y = float(center_x), float(center_y)
+    v.near_plane, v.far_plane = float(near_plane), float(far_plane)
+    v.scale_modifier = float(scale_modifier)
+    v.to_chw = 1 if to_chw else 0
+    v.blur_periphery = 1 if blur_periphery else 0
+    v.anti_aliasing = 1 if anti_aliasing else 0
+    v.foveation_radius_px = int(foveation_radius_px)
+    v.blend_width_px = int(blend_width_px)
+    v.clear_output = 1 if clear_output else 0
+    shape = (3, height, width) if to_chw else (height, width, 3)
+    image = torch.empty(shape, dtype=torch.float32, device=device)
+    v.image = image.data_ptr()
+    if blur_periphery:
+        temp = torch.empty(shape, dtype=torch.float32, device=device)
+        keep.append(temp)
+        v.image_temp = temp.data_ptr()
+    return v, image
+
+
+def render_views(scene_args: tuple, view_args: list, share_preprocess: bool, clear_output: bool,
+                 to_chw, blur_periphery, anti_aliasing, foveation_radius_px, blend_width_px):
+    keep: list = []
+    scene, device = scene_struct(*scene_args, keep=keep)
+    views, images = [], []
+    for va in view_args:
+        v, img = view_struct(device, *va, to_chw, blur_periphery, anti_aliasing, foveation_radius_px,
+                             blend_width_px, clear_output, keep=keep)
+        views.append(v)
+        images.append(img)
+    handle = _lib.handle_for(device.index if device.index is not None else torch.cuda.current_device())
+    stream = torch.cuda.current_stream(device).cuda_stream
+    handle.render(scene, views, share_preprocess, stream)
+    del keep
+    return images

@@ -1,0 +1,625 @@
+// Host pipeline and C ABI of libfhtgs.so (declared in include/fhtgs.h).
+//
+// Per frame and view (rasterization/src/inference.cu:49-472 is the reference
+// driver): clear -> tile classification -> preprocess (+ block sums of
+// tiles-touched) -> scan of the block sums; one wait on the stream for the
+// instance and tile counts; then sort clear -> instances (+ digit histograms)
+// -> onesweep passes -> ranges -> output clear -> blend -> upsample.
+// Scratch lives in the handle and only grows, so a steady stream of frames
+// allocates nothing.
+#include "../../include/fhtgs.h"
+#include "fhtgs_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace fhtgs;
+
+namespace {
+
+constexpr size_t kAlign = 256;
+
+size_t align_up(size_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
+
+int end_bit_of(uint32_t n) {
+    int b = 0;
+    while (n) { ++b; n >>= 1; }
+    return b;
+}
+
+void fov_radii(int width, int height, int fr_px, int bw_px, int& R, int& B) {
+    const int gw = (width + kTileL - 1) / kTileL, gh = (height + kTileL - 1) / kTileL;
+    if (fr_px < 0) {
+        const int cap = (int)std::ceil(std::sqrt((double)gw * gw + (double)gh * gh)) + 1;
+        R = cap;
+        B = cap;
+    } else {
+        R = (fr_px + kTileL - 1) / kTileL;
+        const int raw = fr_px - bw_px;
+        B = raw > 0 ? raw / kTileL : 0;
+    }
+}
+
+SortPlan make_plan(int end_bit, int max_bits) {
+    SortPlan p;
+    std::memset(&p, 0, sizeof(p));
+    p.end_bit = end_bit;
+    p.key_bytes = end_bit <= 16 ? 2 : 4;
+    max_bits = std::max(4, std::min(11, max_bits));
+    p.passes = std::max(1, (end_bit + max_bits - 1) / max_bits);
+    const int per = (end_bit + p.passes - 1) / p.passes;
+    int shift = 0;
+    p.bins_max = 0;
+    for (int i = 0; i < p.passes; ++i) {
+        const int b = std::min(per, end_bit - shift);
+        p.bits[i] = std::max(1, b);
+        p.shift[i] = shift;
+        shift += b;
+        p.bins_max = std::max(p.bins_max, 1 << p.bits[i]);
+    }
+    return p;
+}
+
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t cap = 0;
+};
+
+struct ViewSlot {
+    DeviceBuffer fixed;
+    DeviceBuffer inst;
+    DeviceBuffer debug;
+    uint32_t* host_fi = nullptr;  // pinned
+    ViewParams vp;
+    ViewBuffers vb;
+    InstanceBuffers ib;
+    SortPlan plan;
+    int64_t counts[16];
+    int sorted_src = 0;
+    uint32_t* dbg_keys = nullptr;
+    uint32_t* dbg_vals = nullptr;
+    uint32_t n_instances = 0;
+    uint32_t n_active = 0;
+    int64_t n_prims = 0;
+};
+
+}  // namespace
+
+struct fhtgs_context {
+    int device = 0;
+    std::string err;
+    std::vector<ViewSlot> slots;
+    DeviceBuffer sort_scratch;
+    hipEvent_t done = nullptr;
+    bool have_done = false;
+    int keep_unsorted = 0;
+    int digit_bits = 11;
+    int timing = 0;
+    hipEvent_t ev[FHTGS_NUM_STAGES + 1] = {};
+    float stage_ms[FHTGS_NUM_STAGES] = {};
+};
+
+namespace {
+
+int set_err(fhtgs_context* h, int code, const std::string& msg) {
+    if (h) h->err = msg;
+    return code;
+}
+
+#define HCHK(expr)                                                                          \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return set_err(h, FHTGS_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int ensure(fhtgs_context* h, DeviceBuffer& b, size_t bytes) {
+    if (bytes <= b.cap) return FHTGS_OK;
+    size_t want = std::max(bytes, b.cap + b.cap / 4);
+    want = align_up(want);
+    if (b.ptr) {
+        HCHK(hipDeviceSynchronize());
+        HCHK(hipFree(b.ptr));
+        b.ptr = nullptr;
+        b.cap = 0;
+    }
+    hipError_t e = hipMalloc(&b.ptr, want);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        b.ptr = nullptr;
+        return set_err(h, FHTGS_E_OUT_OF_MEMORY,
+                       "device scratch allocation of " + std::to_string(want) + " bytes failed");
+    }
+    b.cap = want;
+    return FHTGS_OK;
+}
+
+struct Carver {
+    char* base;
+    size_t off = 0;
+    explicit Carver(void* b) : base((char*)b) {}
+    template <typename T>
+    T* take(size_t count) {
+        off = align_up(off);
+        T* p = (T*)(base ? base + off : nullptr);
+        off += sizeof(T) * count;
+        return p;
+    }
+};
+
+void carve_fixed(void* base, int64_t N, int n_large, int n_small, ViewBuffers& vb, size_t& bytes) {
+    Carver c(base);
+    const size_t nb = (size_t)((N + kPrimBlock - 1) / kPrimBlock);
+    vb.frame_info = c.take<uint32_t>(FI_WORDS);
+    vb.tile_num = c.take<uint32_t>((size_t)n_large);
+    vb.tile_map = c.take<uint32_t>((size_t)n_small);
+    vb.tile_type = c.take<uint8_t>((size_t)n_small);
+    vb.tile_map_part = c.take<uint32_t>((size_t)n_small);
+    vb.ranges = c.take<uint2>((size_t)n_small);
+    vb.n_touched = c.take<uint32_t>((size_t)N);
+    vb.bounds = c.take<ushort4>((size_t)N);
+    vb.vpmt1 = c.take<float4>((size_t)N);
+    vb.vpmt2 = c.take<float4>((size_t)N);
+    vb.vpmt4 = c.take<float4>((size_t)N);
+    vb.rgba = c.take<float4>((size_t)N);
+    vb.block_sums = c.take<uint32_t>(std::max<size_t>(nb, 1));
+    bytes = align_up(c.off);
+}
+
+void carve_inst(void* base, int64_t I, const SortPlan& plan, InstanceBuffers& ib, size_t& bytes) {
+    Carver c(base);
+    const int64_t cap = std::max<int64_t>(I, 1);
+    const int64_t tiles = (cap + kSortTile - 1) / kSortTile;
+    ib.keys[0] = c.take<char>((size_t)cap * plan.key_bytes);
+    ib.keys[1] = c.take<char>((size_t)cap * plan.key_bytes);
+    ib.vals[0] = c.take<uint32_t>((size_t)cap);
+    ib.vals[1] = c.take<uint32_t>((size_t)cap);
+    ib.hist = c.take<uint32_t>((size_t)kMaxPasses * kMaxBins);
+    ib.tickets = c.take<uint32_t>((size_t)kMaxPasses + 1);
+    ib.status = c.take<uint32_t>((size_t)plan.passes * (size_t)tiles * plan.bins_max);
+    ib.capacity = cap;
+    ib.tiles_cap = tiles;
+    bytes = align_up(c.off);
+}
+
+int validate_view(fhtgs_context* h, const fhtgs_view& v, int idx) {
+    const std::string p = "view " + std::to_string(idx) + ": ";
+    if (v.width <= 0 || v.height <= 0) return set_err(h, FHTGS_E_INVALID, p + "width and height must be positive");
+    if ((int64_t)v.width * v.height * 3 > (int64_t)1 << 31)
+        return set_err(h, FHTGS_E_INVALID, p + "image too large");
+    if (!v.M || !v.VPM || !v.cam_position || !v.visibility_mask || !v.visibility_mask_area_table ||
+        !v.fovea_mask_area_table || !v.image)
+        return set_err(h, FHTGS_E_INVALID, p + "null camera, table or image pointer");
+    if (v.blur_periphery && !v.image_temp)
+        return set_err(h, FHTGS_E_INVALID, p + "blur_periphery needs image_temp");
+    if (v.background_model_type < 0 || v.background_model_type > 3)
+        return set_err(h, FHTGS_E_INVALID, p + "unknown background model type");
+    if (v.background_model_type != 0 && (!v.background_model_data || !v.VPR_inv))
+        return set_err(h, FHTGS_E_INVALID, p + "background model needs data and VPR_inv");
+    if ((v.background_model_type == 2 || v.background_model_type == 3) &&
+        (v.background_width <= 0 || v.background_height <= 0))
+        return set_err(h, FHTGS_E_INVALID, p + "texture background needs its size");
+    if (v.anti_aliasing)
+        return set_err(h, FHTGS_E_UNSUPPORTED, p + "anti-aliased preprocess is not available in this build");
+    if (v.foveation_radius_px >= 0 && v.blend_width_px > v.foveation_radius_px)
+        return set_err(h, FHTGS_E_INVALID, p + "blend width exceeds the foveation radius");
+    return FHTGS_OK;
+}
+
+void fill_params(const fhtgs_scene& sc, const fhtgs_view& v, ViewParams& vp) {
+    std::memset(&vp, 0, sizeof(vp));
+    vp.M = v.M;
+    vp.VPM = v.VPM;
+    vp.VPR_inv = v.VPR_inv;
+    vp.cam = v.cam_position;
+    vp.vis_mask = v.visibility_mask;
+    vp.vis_sat = v.visibility_mask_area_table;
+    vp.fovea_sat = v.fovea_mask_area_table;
+    vp.bg_data = v.background_model_data;
+    vp.bg_type = v.background_model_type;
+    vp.bg_w = v.background_width;
+    vp.bg_h = v.background_height;
+    vp.width = v.width;
+    vp.height = v.height;
+    vp.gw = (v.width + kTileL - 1) / kTileL;
+    vp.gh = (v.height + kTileL - 1) / kTileL;
+    vp.n_large = vp.gw * vp.gh;
+    vp.n_small = vp.n_large * kNumSub;
+    fov_radii(v.width, v.height, v.foveation_radius_px, v.blend_width_px, vp.fovea_r, vp.blend_r);
+    vp.blend_radius_px = vp.blend_r * kTileL;
+    vp.blend_width_px = v.blend_width_px;
+    // rasterization/src/inference.cu:79-105: clamp, then round to a large
+    // tile; the clamp bound is the small-grid size, as in the reference
+    vp.gcx = std::fmax(0.0f, std::fmin(v.gaze_x, (float)(v.width - 1)));
+    vp.gcy = std::fmax(0.0f, std::fmin(v.gaze_y, (float)(v.height - 1)));
+    const int gsw = vp.gw * kStride, gsh = vp.gh * kStride;
+    vp.gtx = std::max(0, std::min(gsw - 1, ((int)vp.gcx + kTileL / 2) / kTileL));
+    vp.gty = std::max(0, std::min(gsh - 1, ((int)vp.gcy + kTileL / 2) / kTileL));
+    vp.active_sh = v.active_sh_bases;
+    vp.total_sh = sc.total_sh_bases;
+    vp.fx = v.focal_x;
+    vp.fy = v.focal_y;
+    vp.cx = v.center_x;
+    vp.cy = v.center_y;
+    vp.near_plane = v.near_plane;
+    vp.far_plane = v.far_plane;
+    vp.scale_mod = v.scale_modifier;
+    vp.to_chw = v.to_chw ? 1 : 0;
+    vp.blur = v.blur_periphery ? 1 : 0;
+    vp.aaa = v.anti_aliasing ? 1 : 0;
+    vp.image_final = v.image;
+    vp.image_blend = v.blur_periphery ? v.image_temp : v.image;
+}
+
+int record(fhtgs_context* h, int stage, hipStream_t s) {
+    if (!h->timing) return FHTGS_OK;
+    HCHK(hipEventRecord(h->ev[stage], s));
+    return FHTGS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fhtgs_version(void) { return "fhtgs-gfx950 0.1"; }
+
+int fhtgs_create(int device, fhtgs_handle* out) {
+    if (!out) return FHTGS_E_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) {
+        (void)hipGetLastError();
+        return FHTGS_E_INVALID;
+    }
+    fhtgs_context* h = new fhtgs_context();
+    h->device = device;
+    int prev = 0;
+    (void)hipGetDevice(&prev);
+    if (hipSetDevice(device) != hipSuccess || hipEventCreateWithFlags(&h->done, hipEventDisableTiming) != hipSuccess) {
+        delete h;
+        return FHTGS_E_HIP;
+    }
+    for (int i = 0; i <= FHTGS_NUM_STAGES; ++i) {
+        if (hipEventCreate(&h->ev[i]) != hipSuccess) {
+            delete h;
+            return FHTGS_E_HIP;
+        }
+    }
+    (void)hipSetDevice(prev);
+    *out = h;
+    return FHTGS_OK;
+}
+
+int fhtgs_destroy(fhtgs_handle h) {
+    if (!h) return FHTGS_OK;
+    int prev = 0;
+    (void)hipGetDevice(&prev);
+    (void)hipSetDevice(h->device);
+    (void)hipDeviceSynchronize();
+    for (auto& s : h->slots) {
+        if (s.fixed.ptr) (void)hipFree(s.fixed.ptr);
+        if (s.inst.ptr) (void)hipFree(s.inst.ptr);
+        if (s.debug.ptr) (void)hipFree(s.debug.ptr);
+        if (s.host_fi) (void)hipHostFree(s.host_fi);
+    }
+    if (h->sort_scratch.ptr) (void)hipFree(h->sort_scratch.ptr);
+    if (h->done) (void)hipEventDestroy(h->done);
+    for (int i = 0; i <= FHTGS_NUM_STAGES; ++i)
+        if (h->ev[i]) (void)hipEventDestroy(h->ev[i]);
+    (void)hipSetDevice(prev);
+    delete h;
+    return FHTGS_OK;
+}
+
+const char* fhtgs_last_error(fhtgs_handle h) { return h ? h->err.c_str() : "null handle"; }
+
+int fhtgs_fovea_radius_tiles(int32_t width, int32_t height, int32_t fr_px, int32_t bw_px,
+                             int32_t* fovea_tiles, int32_t* blend_tiles) {
+    if (width <= 0 || height <= 0 || !fovea_tiles || !blend_tiles) return FHTGS_E_INVALID;
+    int R, B;
+    fov_radii(width, height, fr_px, bw_px, R, B);
+    *fovea_tiles = R;
+    *blend_tiles = B;
+    return FHTGS_OK;
+}
+
+int fhtgs_set_option(fhtgs_handle h, int32_t option, int64_t value) {
+    if (!h) return FHTGS_E_INVALID;
+    switch (option) {
+        case FHTGS_OPT_KEEP_UNSORTED: h->keep_unsorted = value ? 1 : 0; return FHTGS_OK;
+        case FHTGS_OPT_SORT_DIGIT_BITS:
+            if (value < 4 || value > 11) return set_err(h, FHTGS_E_INVALID, "digit bits must be 4..11");
+            h->digit_bits = (int)value;
+            return FHTGS_OK;
+        case FHTGS_OPT_STAGE_TIMING: h->timing = value ? 1 : 0; return FHTGS_OK;
+        default: return set_err(h, FHTGS_E_INVALID, "unknown option");
+    }
+}
+
+int fhtgs_workspace_bytes(fhtgs_handle h, int64_t* bytes) {
+    if (!h || !bytes) return FHTGS_E_INVALID;
+    int64_t t = (int64_t)h->sort_scratch.cap;
+    for (auto& s : h->slots) t += (int64_t)(s.fixed.cap + s.inst.cap + s.debug.cap);
+    *bytes = t;
+    return FHTGS_OK;
+}
+
+int fhtgs_stage_times(fhtgs_handle h, float* ms) {
+    if (!h || !ms) return FHTGS_E_INVALID;
+    for (int i = 0; i < FHTGS_NUM_STAGES; ++i) ms[i] = h->stage_ms[i];
+    return FHTGS_OK;
+}
+
+int fhtgs_render(fhtgs_handle h, const fhtgs_scene* scene, const fhtgs_view* views, int32_t n_views,
+                 int32_t share_preprocess, void* stream_ptr) {
+    if (!h) return FHTGS_E_INVALID;
+    if (!scene || !views || n_views <= 0) return set_err(h, FHTGS_E_INVALID, "null scene/views or no view");
+    if (share_preprocess && n_views > 2)
+        return set_err(h, FHTGS_E_INVALID, "a shared preprocess covers at most two views");
+    const fhtgs_scene& sc = *scene;
+    if (sc.n_primitives < 0 || sc.n_primitives >= ((int64_t)1 << 31))
+        return set_err(h, FHTGS_E_INVALID, "primitive count out of range");
+    if (sc.n_primitives > 0 && (!sc.positions || !sc.scales || !sc.rotations || !sc.opacities || !sc.sh_0 ||
+                                (!sc.sh_rest && sc.total_sh_bases > 0)))
+        return set_err(h, FHTGS_E_INVALID, "null scene tensor");
+    for (int v = 0; v < n_views; ++v) {
+        const int rc = validate_view(h, views[v], v);
+        if (rc) return rc;
+        if (views[v].active_sh_bases > 1 && views[v].active_sh_bases > sc.total_sh_bases + 1)
+            return set_err(h, FHTGS_E_INVALID, "active_sh_bases exceeds the SH coefficients given");
+    }
+    if (share_preprocess) {
+        const fhtgs_view& a = views[0];
+        for (int v = 1; v < n_views; ++v) {
+            const fhtgs_view& b = views[v];
+            if (a.active_sh_bases != b.active_sh_bases || a.near_plane != b.near_plane ||
+                a.far_plane != b.far_plane || a.scale_modifier != b.scale_modifier || a.anti_aliasing != b.anti_aliasing)
+                return set_err(h, FHTGS_E_INVALID, "shared-preprocess views disagree on preprocess parameters");
+        }
+    }
+    hipStream_t s = (hipStream_t)stream_ptr;
+    int prev_dev = 0;
+    HCHK(hipGetDevice(&prev_dev));
+    if (prev_dev != h->device) HCHK(hipSetDevice(h->device));
+    struct Restore {
+        int d, cur;
+        ~Restore() { if (d != cur) (void)hipSetDevice(d); }
+    } restore{prev_dev, h->device};
+
+    if (h->have_done) HCHK(hipStreamWaitEvent(s, h->done, 0));
+    if ((int)h->slots.size() < n_views) h->slots.resize(n_views);
+    const int64_t N = sc.n_primitives;
+    const uint32_t nb = (uint32_t)((N + kPrimBlock - 1) / kPrimBlock);
+
+    // ---- phase 1: tiles, preprocess, scan
+    for (int v = 0; v < n_views; ++v) {
+        ViewSlot& slot = h->slots[v];
+        fill_params(sc, views[v], slot.vp);
+        size_t bytes = 0;
+        carve_fixed(nullptr, N, slot.vp.n_large, slot.vp.n_small, slot.vb, bytes);
+        int rc = ensure(h, slot.fixed, bytes);
+        if (rc) return rc;
+        carve_fixed(slot.fixed.ptr, N, slot.vp.n_large, slot.vp.n_small, slot.vb, bytes);
+        if (!slot.host_fi) HCHK(hipHostMalloc((void**)&slot.host_fi, FI_WORDS * sizeof(uint32_t), hipHostMallocDefault));
+        slot.n_prims = N;
+    }
+    int rc = record(h, 0, s);
+    if (rc) return rc;
+    for (int v = 0; v < n_views; ++v) HCHK(launch_clear_frame(h->slots[v].vp, h->slots[v].vb, s));
+    if ((rc = record(h, 1, s))) return rc;
+    for (int v = 0; v < n_views; ++v) HCHK(launch_tiles(h->slots[v].vp, h->slots[v].vb, s));
+    if ((rc = record(h, 2, s))) return rc;
+    if (share_preprocess && n_views == 2) {
+        ViewParams vps[2] = {h->slots[0].vp, h->slots[1].vp};
+        ViewBuffers vbs[2] = {h->slots[0].vb, h->slots[1].vb};
+        HCHK(launch_preprocess(vps, vbs, 2, sc.positions, sc.scales, sc.rotations, sc.opacities, sc.sh_0,
+                               sc.sh_rest, (uint32_t)N, 0, s));
+    } else {
+        for (int v = 0; v < n_views; ++v)
+            HCHK(launch_preprocess(&h->slots[v].vp, &h->slots[v].vb, 1, sc.positions, sc.scales, sc.rotations,
+                                   sc.opacities, sc.sh_0, sc.sh_rest, (uint32_t)N, 0, s));
+    }
+    if ((rc = record(h, 3, s))) return rc;
+    for (int v = 0; v < n_views; ++v) {
+        ViewSlot& slot = h->slots[v];
+        if (N > 0) HCHK(launch_scan_blocks(slot.vb, nb, s));
+        HCHK(hipMemcpyAsync(slot.host_fi, slot.vb.frame_info, FI_WORDS * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    }
+    if ((rc = record(h, 4, s))) return rc;
+    HCHK(hipStreamSynchronize(s));
+
+    // ---- phase 2: per view instances, sort, ranges, blend, upsample
+    for (int v = 0; v < n_views; ++v) {
+        ViewSlot& slot = h->slots[v];
+        const fhtgs_view& view = views[v];
+        const uint32_t* fi = slot.host_fi;
+        const uint32_t I = N > 0 ? fi[FI_INSTANCES] : 0u;
+        if (I >= (1u << 30)) return set_err(h, FHTGS_E_INVALID, "more than 2^30 tile instances in one view");
+        slot.n_instances = I;
+        slot.n_active = fi[FI_N_ACTIVE];
+        const int end_bit = end_bit_of((uint32_t)slot.vp.n_small + 1u);
+        slot.plan = make_plan(end_bit, h->digit_bits);
+        size_t bytes = 0;
+        carve_inst(nullptr, I, slot.plan, slot.ib, bytes);
+        if ((rc = ensure(h, slot.inst, bytes))) return rc;
+        carve_inst(slot.inst.ptr, I, slot.plan, slot.ib, bytes);
+        const int64_t tiles = ((int64_t)I + kSortTile - 1) / kSortTile;
+        if (h->keep_unsorted) {
+            const size_t db = align_up((size_t)std::max<uint32_t>(I, 1) * 4) * 2;
+            if ((rc = ensure(h, slot.debug, db))) return rc;
+            slot.dbg_keys = (uint32_t*)slot.debug.ptr;
+            slot.dbg_vals = (uint32_t*)((char*)slot.debug.ptr + db / 2);
+        }
+        HCHK(launch_sort_clear(slot.ib, slot.plan, tiles, s));
+        HCHK(launch_instances(slot.vp, slot.vb, slot.ib, slot.plan, (uint32_t)N, s));
+        if (h->keep_unsorted && I > 0) {
+            HCHK(launch_widen_keys(slot.ib.keys[0], slot.plan.key_bytes, slot.dbg_keys, I, s));
+            HCHK(hipMemcpyAsync(slot.dbg_vals, slot.ib.vals[0], (size_t)I * 4, hipMemcpyDeviceToDevice, s));
+        }
+        if (v == 0 && (rc = record(h, 5, s))) return rc;
+        int src = 0;
+        for (int p = 0; p < slot.plan.passes; ++p) {
+            HCHK(launch_sort_pass(slot.ib, slot.plan, p, src, tiles, slot.vb.frame_info, 0u, s));
+            src = 1 - src;
+        }
+        slot.sorted_src = src;
+        if (v == 0 && (rc = record(h, 6, s))) return rc;
+        HCHK(launch_ranges(slot.vb, slot.ib, slot.plan, src, I, s));
+        if (v == 0 && (rc = record(h, 7, s))) return rc;
+        const uint32_t* part = fi + FI_PART;
+        if (view.clear_output) HCHK(launch_clear_image(slot.vp, slot.vb, slot.n_active <= 1 ? 1 : 0, s));
+        HCHK(launch_blend(slot.vp, slot.vb, slot.ib.vals[src], view.K, slot.n_active, s));
+        if (v == 0 && (rc = record(h, 8, s))) return rc;
+        HCHK(launch_upsample(slot.vp, slot.vb, part[3] - part[2], part[5] - part[4], s));
+        if (v == 0 && (rc = record(h, 9, s))) return rc;
+
+        int64_t* c = slot.counts;
+        c[0] = N; c[1] = slot.vp.n_large; c[2] = slot.vp.n_small; c[3] = slot.n_active;
+        c[4] = I; c[5] = -1; c[6] = end_bit; c[7] = slot.plan.key_bytes; c[8] = -1;
+        c[9] = slot.vp.gw; c[10] = slot.vp.gh; c[11] = slot.vp.fovea_r; c[12] = slot.vp.blend_r;
+        c[13] = slot.vp.gtx; c[14] = slot.vp.gty; c[15] = slot.plan.passes;
+    }
+    if ((rc = record(h, 10, s))) return rc;
+    HCHK(hipEventRecord(h->done, s));
+    h->have_done = true;
+    // the reference call returns after its streams finish (inference.cu:468-470)
+    HCHK(hipStreamSynchronize(s));
+    for (int v = 0; v < n_views; ++v) {
+        ViewSlot& slot = h->slots[v];
+        HCHK(hipMemcpy(slot.host_fi, slot.vb.frame_info, FI_WORDS * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        slot.counts[5] = slot.host_fi[FI_VALID];
+        slot.counts[8] = slot.host_fi[FI_N_VISIBLE];
+        if (slot.host_fi[FI_ERROR] & 1u)
+            return set_err(h, FHTGS_E_TABLES,
+                           "tiles-touched from the culling tables under-counts the visible tiles of a Gaussian");
+        uint32_t sort_err = 0;
+        if (slot.n_instances > 0)
+            HCHK(hipMemcpy(&sort_err, slot.ib.tickets + kMaxPasses, sizeof(uint32_t), hipMemcpyDeviceToHost));
+        if (sort_err)
+            return set_err(h, FHTGS_E_HIP, "instance sort detected an inconsistency (look-back or scatter)");
+    }
+    if (h->timing) {
+        for (int i = 0; i < FHTGS_NUM_STAGES - 1; ++i) {
+            float ms = 0.0f;
+            HCHK(hipEventElapsedTime(&ms, h->ev[i], h->ev[i + 1]));
+            h->stage_ms[i] = ms;
+        }
+        float tot = 0.0f;
+        HCHK(hipEventElapsedTime(&tot, h->ev[0], h->ev[10]));
+        h->stage_ms[FHTGS_NUM_STAGES - 1] = tot;
+    }
+    return FHTGS_OK;
+}
+
+int fhtgs_debug_counts(fhtgs_handle h, int32_t view, int64_t* counts) {
+    if (!h || !counts || view < 0 || view >= (int)h->slots.size()) return FHTGS_E_INVALID;
+    std::memcpy(counts, h->slots[view].counts, sizeof(int64_t) * 16);
+    return FHTGS_OK;
+}
+
+int fhtgs_debug_read(fhtgs_handle h, int32_t view, int32_t id, void* dst, int64_t dst_bytes,
+                     int64_t* written) {
+    if (!h || view < 0 || view >= (int)h->slots.size() || !dst) return FHTGS_E_INVALID;
+    ViewSlot& sl = h->slots[view];
+    const int64_t N = sl.n_prims, I = sl.n_instances, na = sl.n_active;
+    const void* src = nullptr;
+    int64_t bytes = 0;
+    bool widen = false;
+    switch (id) {
+        case FHTGS_DBG_TILE_NUM: src = sl.vb.tile_num; bytes = (int64_t)sl.vp.n_large * 4; break;
+        case FHTGS_DBG_TILE_MAP: src = sl.vb.tile_map; bytes = na * 4; break;
+        case FHTGS_DBG_TILE_TYPE: src = sl.vb.tile_type; bytes = na; break;
+        case FHTGS_DBG_TILE_MAP_PART: src = sl.vb.tile_map_part; bytes = na * 4; break;
+        case FHTGS_DBG_PARTITION: src = sl.vb.frame_info + FI_PART; bytes = 24; break;
+        case FHTGS_DBG_N_TOUCHED: src = sl.vb.n_touched; bytes = N * 4; break;
+        case FHTGS_DBG_BOUNDS: src = sl.vb.bounds; bytes = N * 8; break;
+        case FHTGS_DBG_VPMT1: src = sl.vb.vpmt1; bytes = N * 16; break;
+        case FHTGS_DBG_VPMT2: src = sl.vb.vpmt2; bytes = N * 16; break;
+        case FHTGS_DBG_VPMT4: src = sl.vb.vpmt4; bytes = N * 16; break;
+        case FHTGS_DBG_RGBA: src = sl.vb.rgba; bytes = N * 16; break;
+        case FHTGS_DBG_KEYS_UNSORTED:
+            if (!h->keep_unsorted) return set_err(h, FHTGS_E_INVALID, "FHTGS_OPT_KEEP_UNSORTED is off");
+            src = sl.dbg_keys; bytes = I * 4; break;
+        case FHTGS_DBG_VALS_UNSORTED:
+            if (!h->keep_unsorted) return set_err(h, FHTGS_E_INVALID, "FHTGS_OPT_KEEP_UNSORTED is off");
+            src = sl.dbg_vals; bytes = I * 4; break;
+        case FHTGS_DBG_KEYS_SORTED: src = sl.ib.keys[sl.sorted_src]; bytes = I * 4; widen = sl.plan.key_bytes == 2; break;
+        case FHTGS_DBG_VALS_SORTED: src = sl.ib.vals[sl.sorted_src]; bytes = I * 4; break;
+        case FHTGS_DBG_RANGES: src = sl.vb.ranges; bytes = (int64_t)sl.vp.n_small * 8; break;
+        default: return set_err(h, FHTGS_E_INVALID, "unknown debug buffer id");
+    }
+    if (bytes > dst_bytes) return set_err(h, FHTGS_E_INVALID, "destination too small");
+    int prev = 0;
+    HCHK(hipGetDevice(&prev));
+    HCHK(hipSetDevice(h->device));
+    HCHK(hipDeviceSynchronize());
+    if (bytes > 0) {
+        if (widen) {
+            std::vector<uint16_t> tmp((size_t)I);
+            HCHK(hipMemcpy(tmp.data(), src, (size_t)I * 2, hipMemcpyDeviceToHost));
+            uint32_t* o = (uint32_t*)dst;
+            for (int64_t k = 0; k < I; ++k) o[k] = tmp[(size_t)k];
+        } else {
+            HCHK(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost));
+        }
+    }
+    HCHK(hipSetDevice(prev));
+    if (written) *written = bytes;
+    return FHTGS_OK;
+}
+
+int fhtgs_sort_pairs(fhtgs_handle h, void* keys, uint32_t* values, int64_t n, int32_t key_bytes,
+                     int32_t end_bit, void* stream_ptr) {
+    if (!h) return FHTGS_E_INVALID;
+    if (n < 0 || n >= ((int64_t)1 << 30)) return set_err(h, FHTGS_E_INVALID, "n out of range");
+    if (key_bytes != 2 && key_bytes != 4) return set_err(h, FHTGS_E_INVALID, "key_bytes must be 2 or 4");
+    if (end_bit < 1 || end_bit > key_bytes * 8) return set_err(h, FHTGS_E_INVALID, "end_bit out of range");
+    if (n == 0) return FHTGS_OK;
+    if (!keys || !values) return set_err(h, FHTGS_E_INVALID, "null keys or values");
+    hipStream_t s = (hipStream_t)stream_ptr;
+    int prev = 0;
+    HCHK(hipGetDevice(&prev));
+    if (prev != h->device) HCHK(hipSetDevice(h->device));
+    if (h->have_done) HCHK(hipStreamWaitEvent(s, h->done, 0));
+    SortPlan plan = make_plan(end_bit, h->digit_bits);
+    plan.key_bytes = key_bytes;
+    // scratch: alternate keys/values, histograms, tickets, status
+    InstanceBuffers ib;
+    size_t bytes = 0;
+    carve_inst(nullptr, n, plan, ib, bytes);
+    int rc = ensure(h, h->sort_scratch, bytes);
+    if (rc) return rc;
+    carve_inst(h->sort_scratch.ptr, n, plan, ib, bytes);
+    void* alt_keys = ib.keys[1];
+    uint32_t* alt_vals = ib.vals[1];
+    ib.keys[0] = keys;
+    ib.vals[0] = values;
+    ib.keys[1] = alt_keys;
+    ib.vals[1] = alt_vals;
+    const int64_t tiles = (n + kSortTile - 1) / kSortTile;
+    HCHK(launch_sort_clear(ib, plan, tiles, s));
+    HCHK(launch_sort_histogram(ib, plan, 0, (uint32_t)n, s));
+    int src = 0;
+    for (int p = 0; p < plan.passes; ++p) {
+        HCHK(launch_sort_pass(ib, plan, p, src, tiles, nullptr, (uint32_t)n, s));
+        src = 1 - src;
+    }
+    if (src == 1) {
+        HCHK(hipMemcpyAsync(keys, alt_keys, (size_t)n * key_bytes, hipMemcpyDeviceToDevice, s));
+        HCHK(hipMemcpyAsync(values, alt_vals, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+    }
+    HCHK(hipEventRecord(h->done, s));
+    h->have_done = true;
+    HCHK(hipStreamSynchronize(s));
+    uint32_t sort_err = 0;
+    HCHK(hipMemcpy(&sort_err, ib.tickets + kMaxPasses, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (prev != h->device) HCHK(hipSetDevice(prev));
+    if (sort_err) return set_err(h, FHTGS_E_HIP, "sort detected an inconsistency (look-back or scatter)");
+    return FHTGS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,85 @@
+// Device-side vector helpers for the gfx950 kernels. Float arithmetic follows
+// the convention of fhtgs_math.h: no implicit contraction; each "a*b + c" of
+// the reference expressions is one fmaf; in "a*b - c*d" the first product is
+// fused; dot products are left-to-right fmaf chains.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include "fhtgs_math.h"
+#include "fhtgs_internal.h"
+
+namespace fhtgs {
+
+struct v3 { float x, y, z; };
+struct v4 { float x, y, z, w; };
+
+__device__ __forceinline__ v4 ld4(const float4& f) { return v4{f.x, f.y, f.z, f.w}; }
+__device__ __forceinline__ float4 st4(const v4& v) { return make_float4(v.x, v.y, v.z, v.w); }
+
+// helper_math.h:1245-1252
+__device__ __forceinline__ float dot3(v3 a, v3 b) {
+    return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x));
+}
+__device__ __forceinline__ float dot3(v4 a, v3 b) {
+    return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x));
+}
+__device__ __forceinline__ float dot4(v4 a, v4 b) {
+    return fmaf(a.w, b.w, fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)));
+}
+__device__ __forceinline__ v4 mul4(v4 a, v4 b) { return v4{a.x * b.x, a.y * b.y, a.z * b.z, a.w * b.w}; }
+// helper_math.h:1429-1432
+__device__ __forceinline__ v3 cross3(v3 a, v3 b) {
+    return v3{fmaf(a.y, b.z, -(a.z * b.y)), fmaf(a.z, b.x, -(a.x * b.z)), fmaf(a.x, b.y, -(a.y * b.x))};
+}
+// helper_math.h:1306-1310, rsqrtf taken as 1/sqrtf
+__device__ __forceinline__ v3 normalize3(v3 v) {
+    const float inv = 1.0f / sqrtf(dot3(v, v));
+    return v3{v.x * inv, v.y * inv, v.z * inv};
+}
+
+__device__ __forceinline__ v4 load_row(const float* m, int row) {
+    return v4{m[4 * row + 0], m[4 * row + 1], m[4 * row + 2], m[4 * row + 3]};
+}
+
+// kernel_utils.cuh:43-73
+__device__ __forceinline__ bool in_fovea(int radius, float cx, float cy, float gx, float gy) {
+    const float radius_sq = (float)(radius * radius);
+    const float dx = cx - gx, dy = cy - gy;
+    return fmaf(dy, dy, dx * dx) < radius_sq;
+}
+
+__device__ __forceinline__ bool mask_bit(const uint32_t* mask, uint32_t t) {
+    return (mask[t >> 5] & (1u << (t & 31u))) != 0u;
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    return (1ull << lane_id()) - 1ull;
+}
+
+// Wave64 inclusive prefix sum.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    return x;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+
+__device__ __forceinline__ uint32_t atomic_load_agent(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void atomic_store_agent(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+}  // namespace fhtgs

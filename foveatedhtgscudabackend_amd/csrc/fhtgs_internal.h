@@ -1,0 +1,138 @@
+// Internal structures shared by the host pipeline (fhtgs_api.cpp) and the
+// gfx950 kernels. Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fhtgs {
+
+// rasterization/include/config.h:13-45
+constexpr float kTransmittanceThreshold = 1e-4f;
+constexpr float kMaxFragmentAlpha = 1.0f;
+constexpr float kMinAlphaRcp = 255.0f;
+constexpr float kMinAlpha = 1.0f / 255.0f;
+constexpr float kMaxCutoffSq = 11.0825270903f;
+constexpr float kMinAlphaCore = 1.0f / 20.0f;
+constexpr int kTileL = 16;
+constexpr int kTileS = 8;
+constexpr int kStride = 2;
+constexpr int kNumSub = 4;
+constexpr int kTileFovea = 0;
+constexpr int kTilePeriphery = 1;
+constexpr int kTileBlended = 2;
+
+// Sort geometry: 256 threads x 16 keys per onesweep tile.
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kMaxPasses = 4;
+constexpr int kMaxBins = 2048;
+constexpr uint32_t kFlagAggregate = 1u << 30;
+constexpr uint32_t kFlagInclusive = 2u << 30;
+constexpr uint32_t kValueMask = (1u << 30) - 1u;
+// Upper bound on look-back polls of one predecessor status word; reaching it
+// means a predecessor tile never published, and the pass flags an error and
+// moves on instead of waiting forever.
+constexpr uint32_t kMaxSpin = 1u << 24;
+constexpr uint32_t kSortErrorBit = 2u;
+
+constexpr int kPrimBlock = 256;  // preprocess / instance block (shared partition)
+
+// Frame-info words (device, one block per view), read back once per frame.
+enum FrameInfo : int {
+    FI_PART = 0,          // 6 words: (x,y) per tile type
+    FI_N_ACTIVE = 6,
+    FI_INSTANCES = 7,     // I = sum of tiles-touched
+    FI_VALID = 8,         // non-sentinel instances (atomic)
+    FI_ERROR = 9,         // bit 0: tiles-touched under-count, bit 1: sort inconsistency
+    FI_N_VISIBLE = 10,    // preprocess survivors (atomic)
+    FI_WORDS = 16
+};
+
+struct ViewParams {
+    const float* M;
+    const float* VPM;
+    const float* VPR_inv;
+    const float* cam;
+    const uint32_t* vis_mask;
+    const uint32_t* vis_sat;
+    const uint32_t* fovea_sat;
+    const float* bg_data;
+    int bg_type, bg_w, bg_h;
+    int width, height, gw, gh, n_large, n_small;
+    int fovea_r, blend_r, blend_radius_px, blend_width_px;
+    int gtx, gty;
+    float gcx, gcy;
+    int active_sh, total_sh;
+    float fx, fy, cx, cy, near_plane, far_plane, scale_mod;
+    int to_chw, blur, aaa;
+    float* image_final;   // fovea tiles and all upsampled output
+    float* image_blend;   // periphery / blended blend target (temp image when blurring)
+};
+
+// Per-view device buffers for one frame.
+struct ViewBuffers {
+    uint32_t* frame_info;     // FI_WORDS
+    uint32_t* tile_num;       // n_large
+    uint32_t* tile_map;       // n_small (compacted, raster order)
+    uint8_t* tile_type;       // n_small
+    uint32_t* tile_map_part;  // n_small (partitioned by type)
+    uint2* ranges;            // n_small
+    uint32_t* n_touched;      // N
+    ushort4* bounds;          // N
+    float4* vpmt1;            // N
+    float4* vpmt2;            // N
+    float4* vpmt4;            // N
+    float4* rgba;             // N
+    uint32_t* block_sums;     // ceil(N / kPrimBlock), then exclusive offsets
+};
+
+struct SortPlan {
+    int key_bytes;   // 2 or 4
+    int end_bit;
+    int passes;
+    int bits[kMaxPasses];
+    int shift[kMaxPasses];
+    int bins_max;    // max over passes of 1 << bits
+};
+
+struct InstanceBuffers {
+    void* keys[2];
+    uint32_t* vals[2];
+    uint32_t* hist;      // kMaxPasses * kMaxBins
+    uint32_t* status;    // passes * tiles * bins_max
+    uint32_t* tickets;   // kMaxPasses tickets, then one error word
+    int64_t capacity;    // instances
+    int64_t tiles_cap;   // onesweep tiles
+};
+
+// ---- launchers (one translation unit per kernel family) ----
+hipError_t launch_clear_frame(const ViewParams& vp, const ViewBuffers& vb, hipStream_t s);
+hipError_t launch_tiles(const ViewParams& vp, const ViewBuffers& vb, hipStream_t s);
+hipError_t launch_preprocess(const ViewParams* vps, const ViewBuffers* vbs, int n_views,
+                             const float* pos, const float* scl, const float* rot,
+                             const float* opa, const float* sh0, const float* shr,
+                             uint32_t n, int K_unused, hipStream_t s);
+hipError_t launch_scan_blocks(const ViewBuffers& vb, uint32_t n_blocks, hipStream_t s);
+hipError_t launch_sort_clear(const InstanceBuffers& ib, const SortPlan& plan, int64_t tiles,
+                             hipStream_t s);
+hipError_t launch_instances(const ViewParams& vp, const ViewBuffers& vb, const InstanceBuffers& ib,
+                            const SortPlan& plan, uint32_t n, hipStream_t s);
+hipError_t launch_sort_pass(const InstanceBuffers& ib, const SortPlan& plan, int pass, int src,
+                            int64_t n_tiles, const uint32_t* frame_info, uint32_t n_explicit,
+                            hipStream_t s);
+hipError_t launch_sort_histogram(const InstanceBuffers& ib, const SortPlan& plan, int src,
+                                 uint32_t n, hipStream_t s);
+hipError_t launch_ranges(const ViewBuffers& vb, const InstanceBuffers& ib, const SortPlan& plan,
+                         int src, uint32_t n, hipStream_t s);
+hipError_t launch_blend(const ViewParams& vp, const ViewBuffers& vb, const uint32_t* sorted_vals,
+                        int K, uint32_t n_active, hipStream_t s);
+hipError_t launch_upsample(const ViewParams& vp, const ViewBuffers& vb, uint32_t n_periphery,
+                           uint32_t n_blended, hipStream_t s);
+hipError_t launch_clear_image(const ViewParams& vp, const ViewBuffers& vb, int clear_all,
+                              hipStream_t s);
+hipError_t launch_widen_keys(const void* keys, int key_bytes, uint32_t* out, uint32_t n,
+                             hipStream_t s);
+
+}  // namespace fhtgs

@@ -1,0 +1,277 @@
+// K-buffer hybrid-transparency blend (rasterization/include/kernels/inference.cuh:138-286).
+//
+// One 64-lane workgroup (a single wave) per active 8x8 small tile, any tile
+// type in one launch: FOVEA and BLENDED tiles are full resolution, PERIPHERY
+// tiles cover their 16x16 large tile at stride 2 (inference.cuh:166-169).
+// Gaussians of the tile's range are staged through LDS in batches of 64; per
+// pixel the K nearest fragments with alpha >= 0.05 are kept sorted in
+// registers with 8-bit quantised colour, everything else is accumulated
+// order-independently in the tail; resolve, optional background, saturate.
+#include "fhtgs_device.h"
+
+namespace fhtgs {
+
+// sh_utils.cuh:9-30 (background SH)
+constexpr float B_C0 = 0.28209479177387814f;
+constexpr float B_C1 = 0.48860251190291987f;
+constexpr float B_C2a = 1.0925484305920792f;
+constexpr float B_C2b = 0.94617469575755997f;
+constexpr float B_C2c = 0.31539156525251999f;
+constexpr float B_C2d = 0.54627421529603959f;
+constexpr float B_C3a = 0.59004358992664352f;
+constexpr float B_C3b = 1.7701307697799304f;
+constexpr float B_C3c = 2.8906114426405538f;
+constexpr float B_C3d = 0.45704579946446572f;
+constexpr float B_C3e = 2.2852289973223288f;
+constexpr float B_C3f = 1.865881662950577f;
+constexpr float B_C3g = 1.1195289977703462f;
+constexpr float B_C3h = 1.4453057213202769f;
+
+__device__ __forceinline__ v3 pixel_dir(const ViewParams& vp, float px, float py) {
+    const v4 pc{px, py, 1.0f, 1.0f};
+    const v3 t{dot4(load_row(vp.VPR_inv, 0), pc), dot4(load_row(vp.VPR_inv, 1), pc),
+               dot4(load_row(vp.VPR_inv, 2), pc)};
+    return normalize3(t);
+}
+
+// background_model.cuh:9-48
+__device__ v3 sh_background(const ViewParams& vp, float px, float py) {
+    const v3 t = pixel_dir(vp, px, py);
+    if (t.y > 0.1f) return v3{0.0f, 0.0f, 0.0f};
+    const v3 n = normalize3(t);
+    const float x = n.x, y = n.y, z = n.z;
+    const float xx = x * x, yy = y * y, zz = z * z;
+    const float xy = x * y, xz = x * z, yz = y * z;
+    const float* k = vp.bg_data;
+    const float f[15] = {
+        -(B_C1 * y), B_C1 * z, -(B_C1 * x),
+        B_C2a * xy, -(B_C2a * yz), fmaf(B_C2b, zz, -B_C2c), -(B_C2a * xz), B_C2d * (xx - yy),
+        y * fmaf(B_C3a, yy, -(B_C3b * xx)), (B_C3c * xy) * z, y * fmaf(-B_C3e, zz, B_C3d),
+        z * fmaf(B_C3f, zz, -B_C3g), x * fmaf(-B_C3e, zz, B_C3d), (B_C3h * z) * (xx - yy),
+        x * fmaf(B_C3b, yy, -(B_C3a * xx))};
+    float r0 = fmaf(B_C0, k[0], 0.5f), r1 = fmaf(B_C0, k[1], 0.5f), r2 = fmaf(B_C0, k[2], 0.5f);
+#pragma unroll
+    for (int j = 0; j < 15; ++j) {
+        r0 = fmaf(f[j], k[(j + 1) * 3 + 0], r0);
+        r1 = fmaf(f[j], k[(j + 1) * 3 + 1], r1);
+        r2 = fmaf(f[j], k[(j + 1) * 3 + 2], r2);
+    }
+    v3 col{fh_saturate(fmaf(fh_tanhf(r0), 0.5f, 0.5f)), fh_saturate(fmaf(fh_tanhf(r1), 0.5f, 0.5f)),
+           fh_saturate(fmaf(fh_tanhf(r2), 0.5f, 0.5f))};
+    if (t.y > 0.0f) {
+        const float s = 1.0f - t.y / 0.1f;
+        col = v3{s * col.x, s * col.y, s * col.z};
+    }
+    return col;
+}
+
+// background_model.cuh:52-117
+__device__ v3 tex_background(const ViewParams& vp, float px, float py, bool activate) {
+    const int W = vp.bg_w, H = vp.bg_h;
+    const v3 t = pixel_dir(vp, px, py);
+    if (t.y > 0.1f) return v3{0.0f, 0.0f, 0.0f};
+    const float pi = 0x1.921fb6p+1f;
+    const float ex = fmaf(fh_atan2f(t.x, t.z) / pi, 0.5f, 0.5f);
+    const float ey = fmaf(fh_asinf(t.y) / (0.5f * pi), 0.5f, 0.5f);
+    const float tx = fmaf(ex, (float)W, -0.5f);
+    const float ty = fmaf(ey, (float)H, -0.5f);
+    const int x0 = fh_f2i_rd(tx), y0 = fh_f2i_rd(ty);
+    const int x1 = x0 + 1, y1 = y0 + 1;
+    const float fx = tx - (float)x0, fy = ty - (float)y0;
+    const int x0w = x0 < 0 ? x0 + W : x0;
+    const int x1w = x1 >= W ? x1 - W : x1;
+    const int y0c = max(0, min(H - 1, y0));
+    const int y1c = max(0, min(H - 1, y1));
+    const int64_t i00 = x0w + (int64_t)y0c * W, i10 = x1w + (int64_t)y0c * W;
+    const int64_t i01 = x0w + (int64_t)y1c * W, i11 = x1w + (int64_t)y1c * W;
+    const int64_t nt = (int64_t)W * H;
+    float c[3];
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+        const float* d = vp.bg_data + ch * nt;
+        const float c0 = fmaf(d[i00], 1.0f - fx, d[i10] * fx);
+        const float c1 = fmaf(d[i01], 1.0f - fx, d[i11] * fx);
+        float val = fmaf(c0, 1.0f - fy, c1 * fy);
+        if (activate) val = 1.0f / (1.0f + fh_expf(-val));
+        c[ch] = val;
+    }
+    v3 col{c[0], c[1], c[2]};
+    if (t.y > 0.0f) {
+        const float s = 1.0f - t.y / 0.1f;
+        col = v3{s * col.x, s * col.y, s * col.z};
+    }
+    return col;
+}
+
+// FragmentInfo::pack_channel (inference.cuh:20-22)
+__device__ __forceinline__ uint32_t pack_channel(float x) {
+    return (uint32_t)fmaf(fminf(fmaxf(x, 0.0f), 1.0f), 255.0f, 0.5f);
+}
+
+struct BlendArgs {
+    const uint32_t* frame_info;
+    const uint32_t* tile_map_part;
+    const uint2* ranges;
+    const uint32_t* vals;
+    const float4* vpmt1;
+    const float4* vpmt2;
+    const float4* vpmt4;
+    const float4* rgba;
+};
+
+template <int K>
+__global__ __launch_bounds__(64) void blend_kernel(ViewParams vp, BlendArgs a) {
+    __shared__ float4 sV1[64], sV2[64], sV4[64], sC[64];
+    const uint32_t g = blockIdx.x;
+    const uint32_t* fi = a.frame_info;
+    int type;
+    if (g >= fi[0] && g < fi[1]) type = kTileFovea;
+    else if (g >= fi[2] && g < fi[3]) type = kTilePeriphery;
+    else if (g >= fi[4] && g < fi[5]) type = kTileBlended;
+    else return;
+    const uint32_t tg = a.tile_map_part[g];
+    const uint32_t lt = tg / kNumSub, st = tg % kNumSub;
+    const uint32_t ltx = lt % (uint32_t)vp.gw, lty = lt / (uint32_t)vp.gw;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t step = type == kTilePeriphery ? kStride : 1u;
+    const uint32_t px = ltx * kTileL + (st % kStride) * kTileS + (lane & 7u) * step;
+    const uint32_t py = lty * kTileL + (st / kStride) * kTileS + (lane >> 3) * step;
+    const bool inside = px < (uint32_t)vp.width && py < (uint32_t)vp.height;
+    const float pxf = (float)px, pyf = (float)py;
+
+    float cd[K];
+    uint32_t cc[K];
+#pragma unroll
+    for (int c = 0; c < K; ++c) { cd[c] = 3.402823466e+38f; cc[c] = 0u; }
+    float T_tail = 1.0f;
+    float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f, t3 = 0.0f;
+
+    const uint2 range = a.ranges[tg];
+    for (uint32_t b0 = range.x; b0 < range.y; b0 += 64u) {
+        __syncthreads();
+        const uint32_t idx = b0 + lane;
+        if (idx < range.y) {
+            const uint32_t p = a.vals[idx];
+            sV1[lane] = a.vpmt1[p];
+            sV2[lane] = a.vpmt2[p];
+            sV4[lane] = a.vpmt4[p];
+            sC[lane] = a.rgba[p];
+        }
+        __syncthreads();
+        if (!inside) continue;
+        const int count = (int)min(64u, range.y - b0);
+        for (int j = 0; j < count; ++j) {
+            const float4 V1 = sV1[j], V2 = sV2[j], V4 = sV4[j];
+            const float Px0 = fmaf(-V4.x, pxf, V1.x), Px1 = fmaf(-V4.y, pxf, V1.y);
+            const float Px2 = fmaf(-V4.z, pxf, V1.z), Px3 = fmaf(-V4.w, pxf, V1.w);
+            const float Py0 = fmaf(-V4.x, pyf, V2.x), Py1 = fmaf(-V4.y, pyf, V2.y);
+            const float Py2 = fmaf(-V4.z, pyf, V2.z), Py3 = fmaf(-V4.w, pyf, V2.w);
+            const v3 m{fmaf(Px3, Py0, -(Px0 * Py3)), fmaf(Px3, Py1, -(Px1 * Py3)),
+                       fmaf(Px3, Py2, -(Px2 * Py3))};
+            const v3 d = cross3(v3{Px0, Px1, Px2}, v3{Py0, Py1, Py2});
+            const float num = dot3(m, m);
+            const float den = dot3(d, d);
+            if (num > kMaxCutoffSq * den) continue;
+            const float rcp = 1.0f / den;
+            const float G = fh_expf((-0.5f * num) * rcp);
+            const float4 C = sC[j];
+            const float alpha = fminf(C.w * G, kMaxFragmentAlpha);
+            if (alpha < kMinAlpha) continue;
+            const v3 e0 = cross3(d, m);
+            const v3 e{e0.x * rcp, e0.y * rcp, e0.z * rcp};
+            const float depth = dot3(v3{V4.x, V4.y, V4.z}, e) + V4.w;
+            float p0 = C.x * alpha, p1 = C.y * alpha, p2 = C.z * alpha, p3 = alpha;
+            if (depth < cd[K - 1] && alpha >= kMinAlphaCore) {
+                float fd = depth;
+                uint32_t fc = pack_channel(p0) | (pack_channel(p1) << 8) | (pack_channel(p2) << 16) |
+                              (pack_channel(p3) << 24);
+#pragma unroll
+                for (int c = 0; c < K; ++c) {
+                    if (fd < cd[c]) {
+                        const float td = cd[c];
+                        const uint32_t tc = cc[c];
+                        cd[c] = fd; cc[c] = fc;
+                        fd = td; fc = tc;
+                    }
+                }
+                const float s = 1.0f / 255.0f;
+                p0 = (float)(fc & 0xffu) * s;
+                p1 = (float)((fc >> 8) & 0xffu) * s;
+                p2 = (float)((fc >> 16) & 0xffu) * s;
+                p3 = (float)((fc >> 24) & 0xffu) * s;
+            }
+            const float tt = 1.0f - p3;
+            t0 = t0 + p0; t1 = t1 + p1; t2 = t2 + p2; t3 = t3 + p3;
+            T_tail = T_tail * tt;
+        }
+    }
+    if (!inside) return;
+    float r = 0.0f, gch = 0.0f, bch = 0.0f;
+    float Tc = 1.0f;
+    const float s255 = 1.0f / 255.0f;
+    // the loop condition of inference.cuh:246; once it fails Tc no longer changes
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+        if (!(Tc >= kTransmittanceThreshold)) continue;
+        const uint32_t col = cc[c];
+        const float q0 = (float)(col & 0xffu) * s255, q1 = (float)((col >> 8) & 0xffu) * s255;
+        const float q2 = (float)((col >> 16) & 0xffu) * s255, q3 = (float)((col >> 24) & 0xffu) * s255;
+        r = fmaf(Tc, q0, r);
+        gch = fmaf(Tc, q1, gch);
+        bch = fmaf(Tc, q2, bch);
+        Tc = Tc * (1.0f - q3);
+    }
+    if (Tc >= kTransmittanceThreshold && t3 >= kMinAlpha) {
+        const float wt = Tc * (1.0f - T_tail);
+        const float s = wt * (1.0f / t3);
+        r = fmaf(s, t0, r);
+        gch = fmaf(s, t1, gch);
+        bch = fmaf(s, t2, bch);
+    }
+    if (vp.bg_type != 0) {
+        const float wb = Tc * T_tail;
+        if (wb >= kTransmittanceThreshold) {
+            const v3 bg = vp.bg_type == 1 ? sh_background(vp, pxf, pyf)
+                                          : tex_background(vp, pxf, pyf, vp.bg_type == 3);
+            r = fmaf(wb, bg.x, r);
+            gch = fmaf(wb, bg.y, gch);
+            bch = fmaf(wb, bg.z, bch);
+        }
+    }
+    float* out = type == kTileFovea ? vp.image_final : vp.image_blend;
+    const int64_t idx = (int64_t)vp.width * py + px;
+    if (vp.to_chw) {
+        const int64_t np = (int64_t)vp.width * vp.height;
+        out[idx] = fh_saturate(r);
+        out[np + idx] = fh_saturate(gch);
+        out[2 * np + idx] = fh_saturate(bch);
+    } else {
+        out[3 * idx] = fh_saturate(r);
+        out[3 * idx + 1] = fh_saturate(gch);
+        out[3 * idx + 2] = fh_saturate(bch);
+    }
+}
+
+hipError_t launch_blend(const ViewParams& vp, const ViewBuffers& vb, const uint32_t* sorted_vals,
+                        int K, uint32_t n_active, hipStream_t s) {
+    if (n_active == 0) return hipSuccess;
+    BlendArgs a;
+    a.frame_info = vb.frame_info;
+    a.tile_map_part = vb.tile_map_part;
+    a.ranges = vb.ranges;
+    a.vals = sorted_vals;
+    a.vpmt1 = vb.vpmt1;
+    a.vpmt2 = vb.vpmt2;
+    a.vpmt4 = vb.vpmt4;
+    a.rgba = vb.rgba;
+    // rasterization/src/inference.cu:17-31: largest power of two <= K in 1..32
+    if (K >= 32) blend_kernel<32><<<n_active, 64, 0, s>>>(vp, a);
+    else if (K >= 16) blend_kernel<16><<<n_active, 64, 0, s>>>(vp, a);
+    else if (K >= 8) blend_kernel<8><<<n_active, 64, 0, s>>>(vp, a);
+    else if (K >= 4) blend_kernel<4><<<n_active, 64, 0, s>>>(vp, a);
+    else if (K >= 2) blend_kernel<2><<<n_active, 64, 0, s>>>(vp, a);
+    else blend_kernel<1><<<n_active, 64, 0, s>>>(vp, a);
+    return hipGetLastError();
+}
+
+}  // namespace fhtgs

@@ -1,0 +1,180 @@
+/*
+ * fhtgs.h - C ABI of the MI355X (gfx950) foveated hybrid-transparency
+ * Gaussian-splatting rasterizer (libfhtgs.so).
+ *
+ * Plain C: pointers, sizes and status codes only. Device pointers are HIP
+ * device pointers on the handle's device; `stream` is a hipStream_t (NULL =
+ * the default stream). Every entry point returns FHTGS_OK (0) or a negative
+ * FHTGS_E_* code; fhtgs_last_error() gives the message of the last failure.
+ *
+ * Reference interfaces replaced (paths relative to FoveatedHTGSCudaBackend/):
+ *   fhtgs_render (one view)   <- htgs_foveated::rasterization::inference_wrapper,
+ *                                rasterization/include/inference_api.h:8-38 and
+ *                                rasterization/src/inference_api.cu:13-122
+ *                                (pybind name _C.render, torch_bindings/bindings.cpp:8);
+ *                                with clear_output = 0 it replaces fast_inference_wrapper,
+ *                                rasterization/include/fast_inference_api.h:8-38
+ *                                (_C_benchmarking.benchmark, bindings_benchmarking.cpp:7)
+ *   fhtgs_render (two views,  <- inference_stereo_wrapper,
+ *     shared preprocess)         rasterization/include/inference_stereo_api.h:7-61,
+ *                                rasterization/src/inference_stereo_api.cu:13-192
+ *                                (_C_stereo.render, bindings_stereo.cpp:8)
+ *   fhtgs_workspace_bytes     <- the per-call scratch sizing of required<>/from_blob,
+ *                                rasterization/include/utils/buffer_utils.h:10-143
+ *   fhtgs_sort_pairs          <- cub::DeviceRadixSort::SortPairs over bits [0,end_bit),
+ *                                rasterization/src/inference.cu:267-274
+ *   fhtgs_fovea_radius_tiles  <- config::foveation_radius_tiles / blend_radius_tiles,
+ *                                rasterization/include/config.h:30-35 (made runtime)
+ *   fhtgs_debug_read / fhtgs_debug_counts: stage outputs of the last render
+ *                                (no reference counterpart; parity tests only)
+ */
+#ifndef FHTGS_H
+#define FHTGS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FHTGS_OK 0
+#define FHTGS_E_INVALID (-1)      /* bad argument (shape, pointer, value) */
+#define FHTGS_E_HIP (-2)          /* HIP runtime error */
+#define FHTGS_E_OUT_OF_MEMORY (-3)
+#define FHTGS_E_TABLES (-4)       /* culling tables under-count tiles-touched */
+#define FHTGS_E_UNSUPPORTED (-5)
+
+/* Background models: rasterization/include/utils/rasterization_utils.h:5-10 */
+#define FHTGS_BACKGROUND_NONE 0
+#define FHTGS_BACKGROUND_SH 1          /* 16 x float3 sky SH coefficients */
+#define FHTGS_BACKGROUND_TEXTURE 2     /* (3,h,w) equirectangular, activated */
+#define FHTGS_BACKGROUND_TEXTURE_RAW 3 /* (3,h,w), sigmoid applied after lookup */
+
+/* Tile types: rasterization/include/utils/enums.h:8-13 */
+#define FHTGS_TILE_FOVEA 0
+#define FHTGS_TILE_PERIPHERY 1
+#define FHTGS_TILE_BLENDED 2
+
+/* Reference foveation constants (rasterization/include/config.h:30-32). */
+#define FHTGS_DEFAULT_FOVEATION_RADIUS_PX 375
+#define FHTGS_DEFAULT_BLEND_WIDTH_PX 125
+#define FHTGS_FOVEATION_OFF (-1)
+
+typedef struct fhtgs_context* fhtgs_handle;
+
+/* Scene tensors, all float32 device memory, C-contiguous
+ * (torch_bindings/rasterization.py:73-98 argument order). */
+typedef struct {
+    const float* positions;  /* (N,3) */
+    const float* scales;     /* (N,3) linear standard deviations */
+    const float* rotations;  /* (N,4) quaternion (w,x,y,z), used as given */
+    const float* opacities;  /* (N) or (N,1), activated */
+    const float* sh_0;       /* (N,1,3) */
+    const float* sh_rest;    /* (N,S,3) */
+    int64_t n_primitives;    /* N */
+    int32_t total_sh_bases;  /* S = sh_rest.size(1) */
+    int32_t reserved;
+} fhtgs_scene;
+
+/* One view: the 21 RasterizerSettings fields (torch_bindings/rasterization.py:22-68)
+ * plus the flags and the output buffers the torch wrapper allocates. */
+typedef struct {
+    const float* M;            /* device, >= 3 rows of 4, row major (rows 0..2 used) */
+    const float* VPM;          /* device, 4x4 row major, world -> screen */
+    const float* VPR_inv;      /* device, 4x4 row major, screen -> world directions */
+    const float* cam_position; /* device, 3 */
+    float gaze_x, gaze_y;      /* host values, pixels */
+    const uint32_t* visibility_mask;            /* device, ceil(n_large/32) words */
+    const uint32_t* visibility_mask_area_table; /* device, (gw+1)*(gh+1) */
+    const uint32_t* fovea_mask_area_table;      /* device, (2R+1)^2 */
+    const float* background_model_data;         /* device; SH: 48 floats, texture: (3,h,w) */
+    int32_t background_model_type;
+    int32_t background_width;  /* texture w */
+    int32_t background_height; /* texture h */
+    int32_t K;
+    int32_t active_sh_bases;
+    int32_t width;
+    int32_t height;
+    float focal_x, focal_y, center_x, center_y;
+    float near_plane, far_plane, scale_modifier;
+    int32_t to_chw;
+    int32_t blur_periphery;
+    int32_t anti_aliasing;
+    int32_t foveation_radius_px; /* FHTGS_DEFAULT_FOVEATION_RADIUS_PX; FHTGS_FOVEATION_OFF */
+    int32_t blend_width_px;      /* FHTGS_DEFAULT_BLEND_WIDTH_PX */
+    int32_t clear_output;        /* 1: pixels no tile writes read 0 (render); 0: left as is (benchmark) */
+    float* image;                /* device float32, 3*H*W, layout by to_chw */
+    float* image_temp;           /* device float32, 3*H*W; required iff blur_periphery */
+} fhtgs_view;
+
+int fhtgs_create(int device, fhtgs_handle* out);
+int fhtgs_destroy(fhtgs_handle h);
+const char* fhtgs_last_error(fhtgs_handle h);
+const char* fhtgs_version(void);
+
+/* Renders n_views views of one scene, enqueued on `stream`. With
+ * share_preprocess != 0 the Gaussians are read once for all views (stereo);
+ * the views must then agree on K, active_sh_bases, near/far, scale_modifier
+ * and anti_aliasing (the reference stereo path uses the left eye's values).
+ * The call waits once on the stream for the instance counts, then returns
+ * with the remaining work queued; outputs are valid in stream order. */
+int fhtgs_render(fhtgs_handle h, const fhtgs_scene* scene, const fhtgs_view* views,
+                 int32_t n_views, int32_t share_preprocess, void* stream);
+
+/* Device scratch the handle currently holds (it grows on demand). */
+int fhtgs_workspace_bytes(fhtgs_handle h, int64_t* bytes);
+
+/* Stable LSD radix sort of (key, value) pairs in place on bits [0, end_bit);
+ * key_bytes 2 or 4, values uint32. Uses the handle's scratch. */
+int fhtgs_sort_pairs(fhtgs_handle h, void* keys, uint32_t* values, int64_t n,
+                     int32_t key_bytes, int32_t end_bit, void* stream);
+
+/* Fovea and blend-ring radii in 16-px tiles for an image; the fovea table must
+ * be built for the first. foveation_radius_px < 0 means foveation off. */
+int fhtgs_fovea_radius_tiles(int32_t width, int32_t height, int32_t foveation_radius_px,
+                             int32_t blend_width_px, int32_t* fovea_tiles, int32_t* blend_tiles);
+
+/* Stage outputs of the last fhtgs_render, copied to host memory (synchronous). */
+#define FHTGS_DBG_TILE_NUM 1        /* u32[n_large] */
+#define FHTGS_DBG_TILE_MAP 2        /* u32[n_active] */
+#define FHTGS_DBG_TILE_TYPE 3       /* u8 [n_active] */
+#define FHTGS_DBG_TILE_MAP_PART 4   /* u32[n_active] */
+#define FHTGS_DBG_PARTITION 6       /* u32[6] */
+#define FHTGS_DBG_N_TOUCHED 7       /* u32[N] */
+#define FHTGS_DBG_BOUNDS 8          /* u16[4N] */
+#define FHTGS_DBG_VPMT1 9           /* f32[4N] */
+#define FHTGS_DBG_VPMT2 10
+#define FHTGS_DBG_VPMT4 11
+#define FHTGS_DBG_RGBA 12
+#define FHTGS_DBG_KEYS_UNSORTED 14  /* u32[I] (keys widened to 32 bits) */
+#define FHTGS_DBG_VALS_UNSORTED 15  /* u32[I] */
+#define FHTGS_DBG_KEYS_SORTED 16
+#define FHTGS_DBG_VALS_SORTED 17
+#define FHTGS_DBG_RANGES 18         /* u32[2*n_small] */
+int fhtgs_debug_read(fhtgs_handle h, int32_t view, int32_t id, void* host_dst,
+                     int64_t dst_bytes, int64_t* bytes_written);
+
+/* counts[16]: N, n_large, n_small, n_active, I, n_valid, end_bit, key_bytes,
+ * n_visible(-1: not counted), gw, gh, fovea_tiles, blend_tiles, gaze_tx, gaze_ty, passes */
+int fhtgs_debug_counts(fhtgs_handle h, int32_t view, int64_t* counts);
+
+/* Keep a copy of the unsorted instance arrays for fhtgs_debug_read (costs
+ * one extra copy per frame). */
+#define FHTGS_OPT_KEEP_UNSORTED 1
+/* Maximum radix digit width of the instance sort (8..11, default 11). */
+#define FHTGS_OPT_SORT_DIGIT_BITS 2
+/* Record per-stage HIP events; fhtgs_stage_times returns the last frame's ms. */
+#define FHTGS_OPT_STAGE_TIMING 3
+int fhtgs_set_option(fhtgs_handle h, int32_t option, int64_t value);
+
+/* ms per stage of the last timed frame: clear, tiles, preprocess, scan,
+ * instances, sort, ranges, blend, upsample, total. */
+#define FHTGS_NUM_STAGES 10
+int fhtgs_stage_times(fhtgs_handle h, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FHTGS_H */

@@ -1,0 +1,79 @@
+"""Shared test cases: seeded scenes, cameras and culling tables, prepared for
+both the oracle (numpy) and the GPU path (torch)."""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from foveatedhtgscudabackend_amd import scene as S
+from foveatedhtgscudabackend_amd import tables as T
+
+
+@dataclass
+class Case:
+    name: str
+    n: int
+    width: int
+    height: int
+    fov_deg: float = 60.0
+    cube: float = 10.0
+    seed: int = 0
+    K: int = 8
+    foveation_radius_px: int = 375
+    blend_width_px: int = 125
+    to_chw: bool = False
+    blur: bool = False
+    active_sh: int = 16
+    gaze: tuple | None = None        # pixels; default: seeded within 15 deg
+    invisible_frac: float = 0.0      # random invisible tiles
+    background: int = 0              # 0 none, 1 SH, 2 texture, 3 raw texture
+    scale_modifier: float = 1.0
+    extra: dict = field(default_factory=dict)
+
+    def build(self):
+        sc = S.make_scene(self.n, seed=self.seed, cube=self.cube)
+        cam = S.orbit_camera(self.width, self.height, self.fov_deg, seed=self.seed + 1000)
+        gaze = np.asarray(self.gaze, dtype=np.float32) if self.gaze is not None else S.seeded_gaze(cam, self.seed + 2000)
+        gw, gh = T.grid_size(self.width, self.height)
+        visible = None
+        if self.invisible_frac > 0:
+            rng = np.random.default_rng(self.seed + 3000)
+            visible = rng.random((gh, gw)) >= self.invisible_frac
+        R = T.fovea_radius_tiles_for(self.width, self.height, self.foveation_radius_px, self.blend_width_px)
+        words = T.visibility_mask_np(self.width, self.height, visible)
+        sat = T.visibility_area_table_np(words, self.width, self.height)
+        fsat = T.fovea_area_table_np(R)
+        bg = None
+        if self.background == 1:
+            bg = np.random.default_rng(self.seed + 4000).normal(0.0, 0.5, size=(16, 3)).astype(np.float32)
+        elif self.background in (2, 3):
+            rng = np.random.default_rng(self.seed + 4000)
+            bg = rng.random((3, 32, 64)).astype(np.float32)
+            if self.background == 3:
+                bg = (bg - 0.5) * 6.0
+        return sc, cam, gaze, visible, (words, sat, fsat), bg
+
+    def oracle_view(self, cam, gaze, tables, bg) -> dict:
+        import oracle
+        kw = dict(K=self.K, active_sh_bases=self.active_sh, to_chw=self.to_chw, blur_periphery=self.blur,
+                  foveation_radius_px=self.foveation_radius_px, blend_width_px=self.blend_width_px,
+                  scale_modifier=self.scale_modifier)
+        if bg is not None:
+            kw["background_model_type"] = self.background
+            kw["background_model_data"] = bg
+        return oracle.view_from_camera(cam, gaze, tables, **kw)
+
+    def settings(self, cam, gaze, visible, bg, device="cuda"):
+        from foveatedhtgscudabackend_amd.torch_bindings.rasterization import BackgroundModel
+        background = None
+        if bg is not None:
+            background = (BackgroundModel(self.background), bg)
+        return S.settings_for(cam, gaze, device=device, K=self.K, active_sh_bases=self.active_sh,
+                              foveation_radius_px=self.foveation_radius_px, blend_width_px=self.blend_width_px,
+                              visible=visible, background=background, scale_modifier=self.scale_modifier)
+
+
+def psnr(a: np.ndarray, b: np.ndarray) -> float:
+    mse = float(np.mean((a.astype(np.float64) - b.astype(np.float64)) ** 2))
+    return float("inf") if mse == 0 else 10.0 * np.log10(1.0 / mse)
