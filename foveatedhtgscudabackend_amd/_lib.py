@@ -33,8 +33,8 @@ DBG = {
 OPT_KEEP_UNSORTED = 1
 OPT_SORT_DIGIT_BITS = 2
 OPT_STAGE_TIMING = 3
-NUM_STAGES = 10
-STAGE_NAMES = ("clear", "tiles", "preprocess", "scan", "instances", "sort", "ranges",
+NUM_STAGES = 11
+STAGE_NAMES = ("clear", "tiles", "preprocess", "scan", "sync", "instances", "sort", "ranges",
                "blend", "upsample", "total")
 
 EXPORTED_SYMBOLS = (

@@ -432,10 +432,12 @@ int fhtgs_render(fhtgs_handle h, const fhtgs_scene* scene, const fhtgs_view* vie
     if ((rc = record(h, 4, s))) return rc;
     HCHK(hipStreamSynchronize(s));
 
-    // ---- phase 2: per view instances, sort, ranges, blend, upsample
+    // ---- phase 2: instances, sort, ranges, blend, upsample; each stage for
+    // every view before the next, so one event pair times a stage
+    if ((rc = record(h, 5, s))) return rc;
+    std::vector<int64_t> tiles_of(n_views, 0);
     for (int v = 0; v < n_views; ++v) {
         ViewSlot& slot = h->slots[v];
-        const fhtgs_view& view = views[v];
         const uint32_t* fi = slot.host_fi;
         const uint32_t I = N > 0 ? fi[FI_INSTANCES] : 0u;
         if (I >= (1u << 30)) return set_err(h, FHTGS_E_INVALID, "more than 2^30 tile instances in one view");
@@ -447,41 +449,56 @@ int fhtgs_render(fhtgs_handle h, const fhtgs_scene* scene, const fhtgs_view* vie
         carve_inst(nullptr, I, slot.plan, slot.ib, bytes);
         if ((rc = ensure(h, slot.inst, bytes))) return rc;
         carve_inst(slot.inst.ptr, I, slot.plan, slot.ib, bytes);
-        const int64_t tiles = ((int64_t)I + kSortTile - 1) / kSortTile;
+        tiles_of[v] = ((int64_t)I + kSortTile - 1) / kSortTile;
         if (h->keep_unsorted) {
             const size_t db = align_up((size_t)std::max<uint32_t>(I, 1) * 4) * 2;
             if ((rc = ensure(h, slot.debug, db))) return rc;
             slot.dbg_keys = (uint32_t*)slot.debug.ptr;
             slot.dbg_vals = (uint32_t*)((char*)slot.debug.ptr + db / 2);
         }
-        HCHK(launch_sort_clear(slot.ib, slot.plan, tiles, s));
-        HCHK(launch_instances(slot.vp, slot.vb, slot.ib, slot.plan, (uint32_t)N, s));
-        if (h->keep_unsorted && I > 0) {
-            HCHK(launch_widen_keys(slot.ib.keys[0], slot.plan.key_bytes, slot.dbg_keys, I, s));
-            HCHK(hipMemcpyAsync(slot.dbg_vals, slot.ib.vals[0], (size_t)I * 4, hipMemcpyDeviceToDevice, s));
-        }
-        if (v == 0 && (rc = record(h, 5, s))) return rc;
-        int src = 0;
-        for (int p = 0; p < slot.plan.passes; ++p) {
-            HCHK(launch_sort_pass(slot.ib, slot.plan, p, src, tiles, slot.vb.frame_info, 0u, s));
-            src = 1 - src;
-        }
-        slot.sorted_src = src;
-        if (v == 0 && (rc = record(h, 6, s))) return rc;
-        HCHK(launch_ranges(slot.vb, slot.ib, slot.plan, src, I, s));
-        if (v == 0 && (rc = record(h, 7, s))) return rc;
-        const uint32_t* part = fi + FI_PART;
-        if (view.clear_output) HCHK(launch_clear_image(slot.vp, slot.vb, slot.n_active <= 1 ? 1 : 0, s));
-        HCHK(launch_blend(slot.vp, slot.vb, slot.ib.vals[src], view.K, slot.n_active, s));
-        if (v == 0 && (rc = record(h, 8, s))) return rc;
-        HCHK(launch_upsample(slot.vp, slot.vb, part[3] - part[2], part[5] - part[4], s));
-        if (v == 0 && (rc = record(h, 9, s))) return rc;
-
         int64_t* c = slot.counts;
         c[0] = N; c[1] = slot.vp.n_large; c[2] = slot.vp.n_small; c[3] = slot.n_active;
         c[4] = I; c[5] = -1; c[6] = end_bit; c[7] = slot.plan.key_bytes; c[8] = -1;
         c[9] = slot.vp.gw; c[10] = slot.vp.gh; c[11] = slot.vp.fovea_r; c[12] = slot.vp.blend_r;
         c[13] = slot.vp.gtx; c[14] = slot.vp.gty; c[15] = slot.plan.passes;
+    }
+    for (int v = 0; v < n_views; ++v) {
+        ViewSlot& slot = h->slots[v];
+        HCHK(launch_sort_clear(slot.ib, slot.plan, tiles_of[v], s));
+        HCHK(launch_instances(slot.vp, slot.vb, slot.ib, slot.plan, (uint32_t)N, s));
+        if (h->keep_unsorted && slot.n_instances > 0) {
+            HCHK(launch_widen_keys(slot.ib.keys[0], slot.plan.key_bytes, slot.dbg_keys, slot.n_instances, s));
+            HCHK(hipMemcpyAsync(slot.dbg_vals, slot.ib.vals[0], (size_t)slot.n_instances * 4,
+                                hipMemcpyDeviceToDevice, s));
+        }
+    }
+    if ((rc = record(h, 6, s))) return rc;
+    for (int v = 0; v < n_views; ++v) {
+        ViewSlot& slot = h->slots[v];
+        int src = 0;
+        for (int p = 0; p < slot.plan.passes; ++p) {
+            HCHK(launch_sort_pass(slot.ib, slot.plan, p, src, tiles_of[v], slot.vb.frame_info, 0u, s));
+            src = 1 - src;
+        }
+        slot.sorted_src = src;
+    }
+    if ((rc = record(h, 7, s))) return rc;
+    for (int v = 0; v < n_views; ++v) {
+        ViewSlot& slot = h->slots[v];
+        HCHK(launch_ranges(slot.vb, slot.ib, slot.plan, slot.sorted_src, slot.n_instances, s));
+    }
+    if ((rc = record(h, 8, s))) return rc;
+    for (int v = 0; v < n_views; ++v) {
+        ViewSlot& slot = h->slots[v];
+        if (views[v].clear_output)
+            HCHK(launch_clear_image(slot.vp, slot.vb, slot.n_active <= 1 ? 1 : 0, s));
+        HCHK(launch_blend(slot.vp, slot.vb, slot.ib.vals[slot.sorted_src], views[v].K, slot.n_active, s));
+    }
+    if ((rc = record(h, 9, s))) return rc;
+    for (int v = 0; v < n_views; ++v) {
+        ViewSlot& slot = h->slots[v];
+        const uint32_t* part = slot.host_fi + FI_PART;
+        HCHK(launch_upsample(slot.vp, slot.vb, part[3] - part[2], part[5] - part[4], s));
     }
     if ((rc = record(h, 10, s))) return rc;
     HCHK(hipEventRecord(h->done, s));
@@ -509,7 +526,7 @@ int fhtgs_render(fhtgs_handle h, const fhtgs_scene* scene, const fhtgs_view* vie
             h->stage_ms[i] = ms;
         }
         float tot = 0.0f;
-        HCHK(hipEventElapsedTime(&tot, h->ev[0], h->ev[10]));
+        HCHK(hipEventElapsedTime(&tot, h->ev[0], h->ev[FHTGS_NUM_STAGES - 1]));
         h->stage_ms[FHTGS_NUM_STAGES - 1] = tot;
     }
     return FHTGS_OK;

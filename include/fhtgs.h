@@ -117,8 +117,9 @@ const char* fhtgs_version(void);
  * share_preprocess != 0 the Gaussians are read once for all views (stereo);
  * the views must then agree on K, active_sh_bases, near/far, scale_modifier
  * and anti_aliasing (the reference stereo path uses the left eye's values).
- * The call waits once on the stream for the instance counts, then returns
- * with the remaining work queued; outputs are valid in stream order. */
+ * The call waits on the stream twice: for the instance and tile counts after
+ * the preprocess, and at the end (the reference call also returns only after
+ * its streams finish, rasterization/src/inference.cu:468-470). */
 int fhtgs_render(fhtgs_handle h, const fhtgs_scene* scene, const fhtgs_view* views,
                  int32_t n_views, int32_t share_preprocess, void* stream);
 
@@ -168,9 +169,10 @@ int fhtgs_debug_counts(fhtgs_handle h, int32_t view, int64_t* counts);
 #define FHTGS_OPT_STAGE_TIMING 3
 int fhtgs_set_option(fhtgs_handle h, int32_t option, int64_t value);
 
-/* ms per stage of the last timed frame: clear, tiles, preprocess, scan,
- * instances, sort, ranges, blend, upsample, total. */
-#define FHTGS_NUM_STAGES 10
+/* ms per stage of the last timed frame, all views of the call together:
+ * clear, tiles, preprocess, scan (+ count readback), sync (host wait for the
+ * counts), instances, sort, ranges, blend (+ output clear), upsample, total. */
+#define FHTGS_NUM_STAGES 11
 int fhtgs_stage_times(fhtgs_handle h, float* ms);
 
 #ifdef __cplusplus

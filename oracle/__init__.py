@@ -63,7 +63,8 @@ class OrcView(ctypes.Structure):
                 ("center_y", ctypes.c_float), ("near_plane", ctypes.c_float), ("far_plane", ctypes.c_float),
                 ("scale_modifier", ctypes.c_float), ("to_chw", ctypes.c_int32),
                 ("blur_periphery", ctypes.c_int32), ("anti_aliasing", ctypes.c_int32),
-                ("foveation_radius_px", ctypes.c_int32), ("blend_width_px", ctypes.c_int32)]
+                ("foveation_radius_px", ctypes.c_int32), ("blend_width_px", ctypes.c_int32),
+                ("blend_tile_stride", ctypes.c_int32)]
 
 
 _lib = None
@@ -102,6 +103,8 @@ def load() -> ctypes.CDLL:
         getattr(lib, f).restype = ctypes.c_float
     lib.orc_atan2f.argtypes = [ctypes.c_float, ctypes.c_float]
     lib.orc_atan2f.restype = ctypes.c_float
+    lib.orc_sh_eval.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]
+    lib.orc_sh_eval.restype = None
     _lib = lib
     return lib
 
@@ -158,6 +161,7 @@ def run(scene: dict, view: dict, stop_after: int = 0, want=None) -> dict:
     v.anti_aliasing = int(bool(view.get("anti_aliasing", False)))
     v.foveation_radius_px = int(view.get("foveation_radius_px", 375))
     v.blend_width_px = int(view.get("blend_width_px", 125))
+    v.blend_tile_stride = int(view.get("blend_tile_stride", 1))
     err = ctypes.create_string_buffer(512)
     r = lib.orc_run(ctypes.byref(sc), ctypes.byref(v), int(stop_after), err, 512)
     if not r:

@@ -163,23 +163,17 @@ struct orc_result {
 
 namespace {
 
-// rasterization/include/utils/sh_utils.cuh:32-69
-void sh_to_color(const orc_scene& sc, const orc_view& v, int i, float* out) {
-    const float* s0 = sc.sh_0 + 3 * (int64_t)i;
+// rasterization/include/utils/sh_utils.cuh:32-69, for a unit direction n
+void sh_eval(const float* s0, const float* k, const float* n, int active, float* out) {
     float r[3];
     for (int c = 0; c < 3; ++c) r[c] = fma_(C0, s0[c], 0.5f);
-    if (v.active_sh_bases > 1) {
-        const float* k = sc.sh_rest + (int64_t)i * sc.total_sh_bases * 3;
-        const float* p = sc.positions + 3 * (int64_t)i;
-        float d[3] = {p[0] - v.cam_position[0], p[1] - v.cam_position[1], p[2] - v.cam_position[2]};
-        float n[3];
-        normalize3(d, n);
+    if (active > 1) {
         const float x = n[0], y = n[1], z = n[2];
         float t;
         t = C1 * y; for (int c = 0; c < 3; ++c) r[c] = fma_(-t, k[0 * 3 + c], r[c]);
         t = C1 * z; for (int c = 0; c < 3; ++c) r[c] = fma_(t, k[1 * 3 + c], r[c]);
         t = C1 * x; for (int c = 0; c < 3; ++c) r[c] = fma_(-t, k[2 * 3 + c], r[c]);
-        if (v.active_sh_bases > 4) {
+        if (active > 4) {
             const float xx = x * x, yy = y * y, zz = z * z;
             const float xy = x * y, xz = x * z, yz = y * z;
             t = C2a * xy; for (int c = 0; c < 3; ++c) r[c] = fma_(t, k[3 * 3 + c], r[c]);
@@ -187,7 +181,7 @@ void sh_to_color(const orc_scene& sc, const orc_view& v, int i, float* out) {
             t = fma_(C2b, zz, -C2c); for (int c = 0; c < 3; ++c) r[c] = fma_(t, k[5 * 3 + c], r[c]);
             t = C2a * xz; for (int c = 0; c < 3; ++c) r[c] = fma_(-t, k[6 * 3 + c], r[c]);
             t = C2d * (xx - yy); for (int c = 0; c < 3; ++c) r[c] = fma_(t, k[7 * 3 + c], r[c]);
-            if (v.active_sh_bases > 9) {
+            if (active > 9) {
                 t = y * fma_(C3a, yy, -(C3b * xx)); for (int c = 0; c < 3; ++c) r[c] = fma_(t, k[8 * 3 + c], r[c]);
                 t = (C3c * xy) * z; for (int c = 0; c < 3; ++c) r[c] = fma_(t, k[9 * 3 + c], r[c]);
                 t = y * fma_(-C3e, zz, C3d); for (int c = 0; c < 3; ++c) r[c] = fma_(t, k[10 * 3 + c], r[c]);
@@ -199,6 +193,18 @@ void sh_to_color(const orc_scene& sc, const orc_view& v, int i, float* out) {
         }
     }
     out[0] = r[0]; out[1] = r[1]; out[2] = r[2];
+}
+
+// sh_utils.cuh:255-258: direction from the camera to the Gaussian mean
+void sh_to_color(const orc_scene& sc, const orc_view& v, int i, float* out) {
+    const float* p = sc.positions + 3 * (int64_t)i;
+    float n[3] = {0.0f, 0.0f, 0.0f};
+    if (v.active_sh_bases > 1) {
+        const float d[3] = {p[0] - v.cam_position[0], p[1] - v.cam_position[1], p[2] - v.cam_position[2]};
+        normalize3(d, n);
+    }
+    sh_eval(sc.sh_0 + 3 * (int64_t)i, sc.sh_rest + (int64_t)i * sc.total_sh_bases * 3, n,
+            v.active_sh_bases, out);
 }
 
 // Rows of VPM (or M) times the Gaussian frame [u v w | mu]:
@@ -489,6 +495,10 @@ float orc_asinf(float x) { return fh_asinf(x); }
 float orc_tanf(float x) { return fh_tanf(x); }
 float orc_tanhf(float x) { return fh_tanhf(x); }
 
+void orc_sh_eval(const float* sh0, const float* sh_rest, const float* dir, int32_t active, float* rgb) {
+    sh_eval(sh0, sh_rest, dir, active, rgb);
+}
+
 void orc_build_visibility_sat(const uint32_t* mask, int32_t gw, int32_t gh, uint32_t* sat) {
     const int W1 = gw + 1;
     for (int x = 0; x <= gw; ++x) sat[x] = 0;
@@ -736,6 +746,7 @@ orc_result* orc_run(const orc_scene* scp, const orc_view* vp, int32_t stop_after
         const bool lowres = type == PERIPHERY;
         float* out = type == FOVEA ? image_final : image;
         for (uint32_t g = px0; g < px1; ++g) {
+            if (v.blend_tile_stride > 1 && g % (uint32_t)v.blend_tile_stride != 0) continue;
             const uint32_t tg = R->tile_map_part[g];
             const uint32_t lt = tg / kNumSub, st = tg % kNumSub;
             const uint32_t ltx = lt % gw, lty = lt / gw, stx = st % kStride, sty = st / kStride;
@@ -824,6 +835,7 @@ orc_result* orc_run(const orc_scene* scp, const orc_view* vp, int32_t stop_after
     if (!v.blur_periphery) {
         // bilinear_interpolation (interpolation.cuh:41-146), in place
         for (uint32_t g = pp0; g < pp1; ++g) {
+            if (v.blend_tile_stride > 1 && g % (uint32_t)v.blend_tile_stride != 0) continue;
             const uint32_t lt = R->tile_map_part[g] / kNumSub;
             const int lx = (int)(lt % gw), ly = (int)(lt / gw);
             const int bx = lx * kTileL, by = ly * kTileL;
@@ -842,6 +854,7 @@ orc_result* orc_run(const orc_scene* scp, const orc_view* vp, int32_t stop_after
         }
         // bilinear_interpolation_blended (interpolation.cuh:149-267), in place
         for (uint32_t g = pb0; g < pb1; ++g) {
+            if (v.blend_tile_stride > 1 && g % (uint32_t)v.blend_tile_stride != 0) continue;
             const uint32_t tg = R->tile_map_part[g];
             const uint32_t lt = tg / kNumSub, st = tg % kNumSub;
             const int lx = (int)(lt % gw), ly = (int)(lt / gw);
@@ -868,6 +881,7 @@ orc_result* orc_run(const orc_scene* scp, const orc_view* vp, int32_t stop_after
     } else {
         // interpolate_and_blur (interpolation.cuh:270-413): temp -> final
         for (uint32_t g = pp0; g < pp1; ++g) {
+            if (v.blend_tile_stride > 1 && g % (uint32_t)v.blend_tile_stride != 0) continue;
             const uint32_t lt = R->tile_map_part[g] / kNumSub;
             const int lx = (int)(lt % gw), ly = (int)(lt / gw);
             const int bx = lx * kTileL - kStride, by = ly * kTileL - kStride;
@@ -884,6 +898,7 @@ orc_result* orc_run(const orc_scene* scp, const orc_view* vp, int32_t stop_after
         }
         // interpolate_and_blur_blended (interpolation.cuh:416-584)
         for (uint32_t g = pb0; g < pb1; ++g) {
+            if (v.blend_tile_stride > 1 && g % (uint32_t)v.blend_tile_stride != 0) continue;
             const uint32_t tg = R->tile_map_part[g];
             const uint32_t lt = tg / kNumSub, st = tg % kNumSub;
             const int lx = (int)(lt % gw), ly = (int)(lt / gw);

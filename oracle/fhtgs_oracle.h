@@ -55,6 +55,7 @@ typedef struct {
     int32_t anti_aliasing;
     int32_t foveation_radius_px;     /* reference: 375; <0 means foveation off */
     int32_t blend_width_px;          /* reference: 125 */
+    int32_t blend_tile_stride;       /* >1: blend/upsample only every n-th active tile (timing samples) */
 } orc_view;
 
 /* Result arrays, fetched by id after orc_run. */
@@ -118,6 +119,8 @@ float orc_atan2f(float y, float x);
 float orc_asinf(float x);
 float orc_tanf(float x);
 float orc_tanhf(float x);
+/* SH colour (degree by active bases) for a unit direction; sh_rest holds 15 x 3. */
+void orc_sh_eval(const float* sh0, const float* sh_rest, const float* dir, int32_t active, float* rgb);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,91 @@
+"""C ABI checks that need no GPU: the library loads, exports every entry
+point include/fhtgs.h declares, and the ctypes mirrors match the C layout."""
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "fhtgs.h")
+
+
+def _declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(fhtgs_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from foveatedhtgscudabackend_amd import _lib
+    lib = _lib.load()
+    declared = _declared_functions()
+    assert len(declared) >= 12
+    for name in declared:
+        assert hasattr(lib, name), f"libfhtgs.so does not export {name}"
+    assert set(declared) == set(_lib.EXPORTED_SYMBOLS)
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\bT (fhtgs_\w+)", out))
+    assert set(declared) <= exported
+
+
+def test_struct_layout_matches_header(tmp_path):
+    import ctypes
+    from foveatedhtgscudabackend_amd import _lib
+    src = tmp_path / "layout.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "fhtgs.h"\n'
+        "int main(void){\n"
+        ' printf("%zu %zu\\n", sizeof(fhtgs_view), sizeof(fhtgs_scene));\n'
+        + "".join(f' printf("%zu\\n", offsetof(fhtgs_view, {f}));\n' for f, _ in _lib.View._fields_)
+        + "".join(f' printf("%zu\\n", offsetof(fhtgs_scene, {f}));\n' for f, _ in _lib.Scene._fields_)
+        + " return 0;}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    vals = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
+    assert vals[0] == ctypes.sizeof(_lib.View) and vals[1] == ctypes.sizeof(_lib.Scene)
+    offs = [getattr(_lib.View, f).offset for f, _ in _lib.View._fields_] + \
+           [getattr(_lib.Scene, f).offset for f, _ in _lib.Scene._fields_]
+    assert vals[2:] == offs
+
+
+def test_fovea_radius_rule_matches_table_builder():
+    from foveatedhtgscudabackend_amd import _lib, tables
+    for w, h, fr, bw in ((1920, 1080, 375, 125), (2064, 2208, 232, 125), (800, 800, -1, 125), (64, 48, 100, 40)):
+        R, B = _lib.fovea_radius_tiles(w, h, fr, bw)
+        assert R == tables.fovea_radius_tiles_for(w, h, fr, bw)
+        if fr >= 0:
+            assert R == (fr + 15) // 16 and B == max(0, fr - bw) // 16
+        else:
+            gw, gh = tables.grid_size(w, h)
+            # the gaze tile is in [0, gw] x [0, gh] and tile centres in
+            # [0.5, gw - 0.5] x [0.5, gh - 0.5]: every centre is within R
+            assert R == B and R * R > gw ** 2 + gh ** 2
+
+
+def test_create_without_gpu_fails_cleanly():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from foveatedhtgscudabackend_amd import _lib
+    with pytest.raises(RuntimeError):
+        _lib.Handle(0)
+
+
+def test_binding_rejects_bad_shapes_before_any_gpu_call():
+    import torch
+    from foveatedhtgscudabackend_amd import _binding
+    keep = []
+    with pytest.raises(ValueError):
+        _binding.scene_struct(torch.zeros(5, 2), torch.zeros(5, 3), torch.zeros(5, 4), torch.zeros(5),
+                              torch.zeros(5, 1, 3), torch.zeros(5, 15, 3), keep=keep)
+    with pytest.raises(ValueError):
+        _binding.scene_struct(torch.zeros(5, 3), torch.zeros(5, 3), torch.zeros(4, 4), torch.zeros(5),
+                              torch.zeros(5, 1, 3), torch.zeros(5, 15, 3), keep=keep)
+    with pytest.raises(ValueError):  # CPU tensors are rejected: no CPU fallback
+        _binding.scene_struct(torch.zeros(5, 3), torch.zeros(5, 3), torch.zeros(5, 4), torch.zeros(5),
+                              torch.zeros(5, 1, 3), torch.zeros(5, 15, 3), keep=keep)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run(["python", os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True)
+    assert r.returncode == 2 and "one GPU" in r.stderr
