@@ -203,8 +203,6 @@ int validate_view(fhtgs_context* h, const fhtgs_view& v, int idx) {
     if ((v.background_model_type == 2 || v.background_model_type == 3) &&
         (v.background_width <= 0 || v.background_height <= 0))
         return set_err(h, FHTGS_E_INVALID, p + "texture background needs its size");
-    if (v.anti_aliasing)
-        return set_err(h, FHTGS_E_UNSUPPORTED, p + "anti-aliased preprocess is not available in this build");
     if (v.foveation_radius_px >= 0 && v.blend_width_px > v.foveation_radius_px)
         return set_err(h, FHTGS_E_INVALID, p + "blend width exceeds the foveation radius");
     return FHTGS_OK;

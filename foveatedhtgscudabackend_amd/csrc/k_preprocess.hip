@@ -94,6 +94,245 @@ __device__ __forceinline__ uint32_t tile_hi(float c, float e, int grid) {
     return min((uint32_t)grid, (uint32_t)t);
 }
 
+// ---- anti-aliased variant (rasterization/include/utils/kernel_utils_aaa.cuh)
+
+__device__ __forceinline__ v4 row_minus(v4 a, v4 b, float s) {
+    return v4{fmaf(-b.x, s, a.x), fmaf(-b.y, s, a.y), fmaf(-b.z, s, a.z), fmaf(-b.w, s, a.w)};
+}
+
+// kernel_utils_aaa.cuh:9-25
+__device__ __forceinline__ float aaa_contrib_plane(v4 pl, v4 V1, v4 V2, v4 V3, v4 V4, v4& ps) {
+    const float norm = pl.w / fmaf(pl.z, pl.z, fmaf(pl.y, pl.y, pl.x * pl.x));
+    const v3 g{(-norm) * pl.x, (-norm) * pl.y, (-norm) * pl.z};
+    const float gw = (-norm) * pl.w;
+    ps = v4{dot3(V1, g) + V1.w, dot3(V2, g) + V2.w, dot3(V3, g) + V3.w, dot3(V4, g) + V4.w};
+    return -gw;
+}
+
+// kernel_utils_aaa.cuh:27-65
+__device__ __forceinline__ float aaa_contrib_ray(v4 a, v4 b, v4 V1, v4 V2, v4 V3, v4 V4, v4& ps) {
+    const v3 d = cross3(v3{a.x, a.y, a.z}, v3{b.x, b.y, b.z});
+    const v3 m{fmaf(a.w, b.x, -(a.x * b.w)), fmaf(a.w, b.y, -(a.y * b.w)), fmaf(a.w, b.z, -(a.z * b.w))};
+    const float dd = dot3(d, d);
+    const v3 mdd{m.x / dd, m.y / dd, m.z / dd};
+    const v3 mp = cross3(d, mdd);
+    ps = v4{dot3(V1, mp) + V1.w, dot3(V2, mp) + V2.w, dot3(V3, mp) + V3.w, dot3(V4, mp) + V4.w};
+    return dot3(m, mdd);
+}
+
+__device__ __forceinline__ bool aaa_in_range(float p, float from, float ext, float ph) {
+    const float d = fmaf(-from, ph, p);
+    return d > 0.0f && d < ext * ph;
+}
+
+// kernel_utils_aaa.cuh:77-133
+__device__ float aaa_contrib_frustum(v4 V1, v4 V2, v4 V3, v4 V4, float width, float height) {
+    float best = 3.402823466e+38f;
+    const float rf0 = 0.0f, rf1 = 0.0f, rf2 = -1.0f;
+    const float ex0 = width - 1.0f, ex1 = height - 1.0f, ex2 = 2.0f;
+    const float d0 = fmaf(-rf0, V4.w, V1.w), d1 = fmaf(-rf1, V4.w, V2.w), d2 = fmaf(-rf2, V4.w, V3.w);
+    const bool bx = d0 > 0.0f && d0 < ex0 * V4.w;
+    const bool by = d1 > 0.0f && d1 < ex1 * V4.w;
+    const bool bz = d2 > 0.0f && d2 < ex2 * V4.w;
+    if (bx && by && bz) return 0.0f;
+    const float dx = copysignf(ex0 * 0.5f, fmaf(-(ex0 * 0.5f), V4.w, d0));
+    const float dy = copysignf(ex1 * 0.5f, fmaf(-(ex1 * 0.5f), V4.w, d1));
+    const v4 cpx = row_minus(V1, V4, fmaf(ex0, 0.5f, rf0) + dx);
+    const v4 cpy = row_minus(V2, V4, fmaf(ex1, 0.5f, rf1) + dy);
+    v4 ps;
+    float c = aaa_contrib_plane(cpx, V1, V2, V3, V4, ps);
+    if (c < best && aaa_in_range(ps.y, rf1, ex1, ps.w) && aaa_in_range(ps.z, rf2, ex2, ps.w)) best = c;
+    c = aaa_contrib_plane(cpy, V1, V2, V3, V4, ps);
+    if (c < best && aaa_in_range(ps.x, rf0, ex0, ps.w) && aaa_in_range(ps.z, rf2, ex2, ps.w)) best = c;
+    c = aaa_contrib_ray(cpx, cpy, V1, V2, V3, V4, ps);
+    if (c < best && aaa_in_range(ps.z, rf2, ex2, ps.w)) best = c;
+    const v4 opy = row_minus(V2, V4, fmaf(ex1, 0.5f, rf1) - dy);
+    c = aaa_contrib_ray(cpx, opy, V1, V2, V3, V4, ps);
+    if (c < best && aaa_in_range(ps.z, rf2, ex2, ps.w)) best = c;
+    const v4 opx = row_minus(V1, V4, fmaf(ex0, 0.5f, rf0) - dx);
+    c = aaa_contrib_ray(opx, cpy, V1, V2, V3, V4, ps);
+    if (c < best && aaa_in_range(ps.z, rf2, ex2, ps.w)) best = c;
+    return best;
+}
+
+// kernel_utils_aaa.cuh:135-144 (fmodf is exact)
+__device__ __forceinline__ float aaa_normalize_angle(float th) {
+    const float pi = 3.141592654f;
+    const float two_pi = 2.0f * pi;
+    th = fmodf(th, two_pi);
+    if (th > pi) th -= two_pi;
+    if (th <= -pi) th += two_pi;
+    return th;
+}
+
+// kernel_utils_aaa.cuh:162-201, one axis: tan of the clamped angle range
+__device__ void aaa_theta(v3 vd, v4 MTa, v4 MT3, float cutoff, bool is_x, float& lo, float& hi) {
+    const float pi = 3.141592654f;
+    const float two_pi = 2.0f * pi;
+    const float max_theta = fmaf(0.5f, pi, -1e-5f);
+    const v4 t{cutoff, cutoff, cutoff, -1.0f};
+    const float mu = fh_atan2f(is_x ? vd.x : vd.y, vd.z);
+    const float sa = dot4(t, mul4(MTa, MTa)), sz = dot4(t, mul4(MT3, MT3)), mid = dot4(t, mul4(MTa, MT3));
+    const float inside = fmaf(mid, mid, -(sz * sa));
+    float r0 = -max_theta, r1 = max_theta;
+    if (inside > 0.0f) {
+        const float sq = sqrtf(inside);
+        float a0 = fh_atan2f(-(mid + sq), -sz);
+        float a1 = fh_atan2f(-(mid - sq), -sz);
+        while (a0 > mu) a0 -= pi;
+        while (a0 < (mu - pi)) a0 += pi;
+        while (a1 < mu) a1 += pi;
+        while (a1 > (mu + pi)) a1 -= pi;
+        const float n0 = aaa_normalize_angle(a0), n1 = aaa_normalize_angle(a1);
+        if (mu < 0.0f && fabsf(n0) < fabsf(n1)) { a0 += two_pi; a1 += two_pi; }
+        else if (mu > 0.0f && fabsf(n1) < fabsf(n0)) { a0 -= two_pi; a1 -= two_pi; }
+        r0 = fmaxf(r0, a0);
+        r1 = fminf(r1, a1);
+    }
+    lo = fh_tanf(r0);
+    hi = fh_tanf(r1);
+}
+
+struct Frame {
+    float R11, R12, R13, R21, R22, R23, R31, R32, R33;
+};
+
+// Culled or not; outputs as the non-AAA path plus the dilated opacity
+// (kernel_utils_aaa.cuh:210-363).
+__device__ bool aaa_transform_cull(const ViewParams& vp, const Frame& F, v3 mu, v3 os, float& opacity,
+                                   uint32_t& b0, uint32_t& b1, uint32_t& b2, uint32_t& b3, v4& V1,
+                                   v4& V2, v4& V4) {
+    const v4 M3 = load_row(vp.M, 2);
+    const float z = dot3(M3, mu) + M3.w;
+    const float max_focal = fmaxf(vp.fx, vp.fy);
+    const float fsv = z / max_focal;
+    const float fvv = (fsv * fsv) * 0.3f;
+    const v3 ov{os.x * os.x, os.y * os.y, os.z * os.z};
+    const v3 var{ov.x + fvv, ov.y + fvv, ov.z + fvv};
+    const v3 s{sqrtf(var.x), sqrtf(var.y), sqrtf(var.z)};
+    const v3 cam{vp.cam[0], vp.cam[1], vp.cam[2]};
+    const v3 dw = normalize3(v3{mu.x - cam.x, mu.y - cam.y, mu.z - cam.z});
+    const v3 dg{fmaf(F.R31, dw.z, fmaf(F.R21, dw.y, F.R11 * dw.x)), fmaf(F.R32, dw.z, fmaf(F.R22, dw.y, F.R12 * dw.x)),
+                fmaf(F.R33, dw.z, fmaf(F.R23, dw.y, F.R13 * dw.x))};
+    const v3 rr{dg.x * dg.x, dg.y * dg.y, dg.z * dg.z};
+    const float det0 = dot3(rr, v3{ov.y * ov.z, ov.z * ov.x, ov.x * ov.y});
+    const float det1 = dot3(rr, v3{var.y * var.z, var.z * var.x, var.x * var.y});
+    opacity = opacity * sqrtf(fh_saturate(det0 / det1));
+    if (opacity < kMinAlpha) return true;
+    const float rho = 2.0f * fh_logf(opacity * kMinAlphaRcp);
+    const v3 cs{cam.x - mu.x, cam.y - mu.y, cam.z - mu.z};
+    const v3 cg{fmaf(F.R31, cs.z, fmaf(F.R21, cs.y, F.R11 * cs.x)) / s.x,
+                fmaf(F.R32, cs.z, fmaf(F.R22, cs.y, F.R12 * cs.x)) / s.y,
+                fmaf(F.R33, cs.z, fmaf(F.R23, cs.y, F.R13 * cs.x)) / s.z};
+    if (dot3(cg, cg) < rho) return true;
+    const float sm = vp.scale_mod;
+    const v3 u{(F.R11 * s.x) * sm, (F.R21 * s.x) * sm, (F.R31 * s.x) * sm};
+    const v3 vv{(F.R12 * s.y) * sm, (F.R22 * s.y) * sm, (F.R32 * s.y) * sm};
+    const v3 w{(F.R13 * s.z) * sm, (F.R23 * s.z) * sm, (F.R33 * s.z) * sm};
+    V1 = frame_row(load_row(vp.VPM, 0), u, vv, w, mu);
+    V2 = frame_row(load_row(vp.VPM, 1), u, vv, w, mu);
+    const v4 V3 = frame_row(load_row(vp.VPM, 2), u, vv, w, mu);
+    V4 = frame_row(load_row(vp.VPM, 3), u, vv, w, mu);
+    if (aaa_contrib_frustum(V1, V2, V3, V4, (float)vp.width, (float)vp.height) > rho) return true;
+    const v4 MT1 = frame_row(load_row(vp.M, 0), u, vv, w, mu);
+    const v4 MT2 = frame_row(load_row(vp.M, 1), u, vv, w, mu);
+    const v3 vd = normalize3(v3{MT1.w, MT2.w, V4.w});
+    float tx0, tx1, ty0, ty1;
+    aaa_theta(vd, MT1, V4, rho, true, tx0, tx1);
+    aaa_theta(vd, MT2, V4, rho, false, ty0, ty1);
+    const float bx0 = fmaf(vp.fx, tx0, vp.cx), bx1 = fmaf(vp.fx, tx1, vp.cx);
+    const float by0 = fmaf(vp.fy, ty0, vp.cy), by1 = fmaf(vp.fy, ty1, vp.cy);
+    const float ccx = (bx1 + bx0) / 2.0f, ccy = (by1 + by0) / 2.0f;
+    const float ecx = (bx1 - bx0) / 2.0f, ecy = (by1 - by0) / 2.0f;
+    // kernel_utils_aaa.cuh:311-316: no half-pixel shift in this variant
+    b0 = min((uint32_t)vp.gw, (uint32_t)max(0, fh_f2i_rd((ccx - ecx) / 16.0f)));
+    b1 = min((uint32_t)vp.gw, (uint32_t)max(0, fh_f2i_ru((ccx + ecx) / 16.0f)));
+    b2 = min((uint32_t)vp.gh, (uint32_t)max(0, fh_f2i_rd((ccy - ecy) / 16.0f)));
+    b3 = min((uint32_t)vp.gh, (uint32_t)max(0, fh_f2i_ru((ccy + ecy) / 16.0f)));
+    return false;
+}
+
+__device__ __forceinline__ uint32_t tiles_touched(const ViewParams& vp, uint32_t b0, uint32_t b1, uint32_t b2,
+                                                  uint32_t b3) {
+    // kernel_utils.cuh:168-210
+    const int fd = 2 * vp.fovea_r;
+    const int tlx = vp.gtx - vp.fovea_r, tly = vp.gty - vp.fovea_r;
+    const uint32_t f0 = (uint32_t)min(fd, max(0, (int)b0 - tlx));
+    const uint32_t f1 = (uint32_t)min(fd, max(0, (int)b1 - tlx));
+    const uint32_t f2 = (uint32_t)min(fd, max(0, (int)b2 - tly));
+    const uint32_t f3 = (uint32_t)min(fd, max(0, (int)b3 - tly));
+    const uint32_t W1 = (uint32_t)vp.gw + 1u;
+    const uint32_t* vs = vp.vis_sat;
+    uint32_t nt = vs[b1 + W1 * b3] + vs[b0 + W1 * b2] - vs[b1 + W1 * b2] - vs[b0 + W1 * b3];
+    if (f1 != f0 && f3 != f2) {
+        const uint32_t F1 = (uint32_t)fd + 1u;
+        const uint32_t* fs = vp.fovea_sat;
+        nt += fs[f1 + F1 * f3] + fs[f0 + F1 * f2] - fs[f1 + F1 * f2] - fs[f0 + F1 * f3];
+    }
+    return nt;
+}
+
+template <int NV>
+__global__ __launch_bounds__(kPrimBlock) void preprocess_aaa_kernel(PreprocessArgs a) {
+    __shared__ uint32_t s_sum[NV][kPrimBlock / 64];
+    __shared__ uint32_t s_vis[NV][kPrimBlock / 64];
+    const uint32_t i = blockIdx.x * kPrimBlock + threadIdx.x;
+    const bool valid = i < a.n;
+    uint32_t touched[NV];
+    bool visible[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) { touched[v] = 0u; visible[v] = false; }
+    if (valid) {
+        const v3 mu{a.pos[3 * i + 0], a.pos[3 * i + 1], a.pos[3 * i + 2]};
+        const v3 os{a.scl[3 * i + 0], a.scl[3 * i + 1], a.scl[3 * i + 2]};
+        const float4 q = *reinterpret_cast<const float4*>(a.rot + 4 * (size_t)i);
+        const float r = q.x, x = q.y, y = q.z, z = q.w;
+        const float xx = x * x, yy = y * y, zz = z * z;
+        const float xy = x * y, xz = x * z, yz = y * z;
+        const float rx = r * x, ry = r * y, rz = r * z;
+        const Frame F{fmaf(-2.0f, yy + zz, 1.0f), 2.0f * (xy - rz), 2.0f * (xz + ry),
+                      2.0f * (xy + rz), fmaf(-2.0f, xx + zz, 1.0f), 2.0f * (yz - rx),
+                      2.0f * (xz - ry), 2.0f * (yz + rx), fmaf(-2.0f, xx + yy, 1.0f)};
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const ViewParams& vp = a.vp[v];
+            float opacity = a.opa[i];
+            uint32_t b0, b1, b2, b3;
+            v4 V1, V2, V4;
+            if (aaa_transform_cull(vp, F, mu, os, opacity, b0, b1, b2, b3, V1, V2, V4)) continue;
+            const uint32_t nt = tiles_touched(vp, b0, b1, b2, b3);
+            if (nt == 0u) continue;
+            touched[v] = nt;
+            visible[v] = true;
+            const ViewBuffers& vb = a.vb[v];
+            vb.bounds[i] = make_ushort4((unsigned short)b0, (unsigned short)b1, (unsigned short)b2, (unsigned short)b3);
+            vb.vpmt1[i] = st4(V1);
+            vb.vpmt2[i] = st4(V2);
+            vb.vpmt4[i] = st4(V4);
+            const v3 cam{vp.cam[0], vp.cam[1], vp.cam[2]};
+            const v3 rgb = sh_color(a.sh0 + 3 * (size_t)i, a.shr + (size_t)i * vp.total_sh * 3, mu, cam,
+                                    vp.active_sh, vp.total_sh);
+            vb.rgba[i] = make_float4(rgb.x, rgb.y, rgb.z, opacity);
+        }
+    }
+    const uint32_t lane = lane_id(), wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        if (valid) a.vb[v].n_touched[i] = touched[v];
+        const uint32_t ws = wave_sum(touched[v]);
+        const uint32_t wv = (uint32_t)__popcll(__ballot(visible[v]));
+        if (lane == 0) { s_sum[v][wid] = ws; s_vis[v][wid] = wv; }
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        const int v = threadIdx.x;
+        uint32_t tot = 0, vis = 0;
+        for (int w = 0; w < kPrimBlock / 64; ++w) { tot += s_sum[v][w]; vis += s_vis[v][w]; }
+        a.vb[v].block_sums[blockIdx.x] = tot;
+        if (vis) atomicAdd(&a.vb[v].frame_info[FI_N_VISIBLE], vis);
+    }
+}
+
 template <int NV>
 __global__ __launch_bounds__(kPrimBlock) void preprocess_kernel(PreprocessArgs a) {
     __shared__ uint32_t s_sum[NV][kPrimBlock / 64];
@@ -217,8 +456,14 @@ hipError_t launch_preprocess(const ViewParams* vps, const ViewBuffers* vbs, int 
     for (int v = 0; v < n_views; ++v) { a.vp[v] = vps[v]; a.vb[v] = vbs[v]; }
     a.pos = pos; a.scl = scl; a.rot = rot; a.opa = opa; a.sh0 = sh0; a.shr = shr; a.n = n;
     const uint32_t blocks = (n + kPrimBlock - 1) / kPrimBlock;
-    if (n_views == 1) preprocess_kernel<1><<<blocks, kPrimBlock, 0, s>>>(a);
-    else preprocess_kernel<2><<<blocks, kPrimBlock, 0, s>>>(a);
+    // inference.cuh:86-112: the anti-aliased variant replaces transform and cull
+    if (vps[0].aaa) {
+        if (n_views == 1) preprocess_aaa_kernel<1><<<blocks, kPrimBlock, 0, s>>>(a);
+        else preprocess_aaa_kernel<2><<<blocks, kPrimBlock, 0, s>>>(a);
+    } else {
+        if (n_views == 1) preprocess_kernel<1><<<blocks, kPrimBlock, 0, s>>>(a);
+        else preprocess_kernel<2><<<blocks, kPrimBlock, 0, s>>>(a);
+    }
     return hipGetLastError();
 }
 

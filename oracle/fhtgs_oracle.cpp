@@ -306,6 +306,219 @@ bool transform_and_cull(const orc_scene& sc, const orc_view& v, int i, int gw, i
     return n == 0;
 }
 
+// ---- anti-aliased variant: rasterization/include/utils/kernel_utils_aaa.cuh
+
+// kernel_utils_aaa.cuh:9-25
+float aaa_contrib_plane(const float* pl, const float* V1, const float* V2, const float* V3,
+                        const float* V4, float* ps) {
+    const float norm = pl[3] / fma_(pl[2], pl[2], fma_(pl[1], pl[1], pl[0] * pl[0]));
+    const float g[4] = {(-norm) * pl[0], (-norm) * pl[1], (-norm) * pl[2], (-norm) * pl[3]};
+    ps[0] = dot3(V1, g) + V1[3];
+    ps[1] = dot3(V2, g) + V2[3];
+    ps[2] = dot3(V3, g) + V3[3];
+    ps[3] = dot3(V4, g) + V4[3];
+    return -g[3];
+}
+
+// kernel_utils_aaa.cuh:27-65
+float aaa_contrib_ray(const float* a, const float* b, const float* V1, const float* V2,
+                      const float* V3, const float* V4, float* ps) {
+    float d[3];
+    cross3(a, b, d);
+    const float m[3] = {fma_(a[3], b[0], -(a[0] * b[3])), fma_(a[3], b[1], -(a[1] * b[3])),
+                        fma_(a[3], b[2], -(a[2] * b[3]))};
+    const float dd = dot3(d, d);
+    const float mdd[3] = {m[0] / dd, m[1] / dd, m[2] / dd};
+    float mp[3];
+    cross3(d, mdd, mp);
+    ps[0] = dot3(V1, mp) + V1[3];
+    ps[1] = dot3(V2, mp) + V2[3];
+    ps[2] = dot3(V3, mp) + V3[3];
+    ps[3] = dot3(V4, mp) + V4[3];
+    return dot3(m, mdd);
+}
+
+// kernel_utils_aaa.cuh:67-75
+inline bool aaa_in_range(float p, float from, float ext, float ph) {
+    const float d = fma_(-from, ph, p);
+    return d > 0.0f && d < ext * ph;
+}
+
+// kernel_utils_aaa.cuh:77-133
+float aaa_contrib_frustum(const float* V1, const float* V2, const float* V3, const float* V4,
+                          float width, float height) {
+    float best = 3.402823466e+38f;
+    const float rf[3] = {0.0f, 0.0f, -1.0f};
+    const float ex[3] = {width - 1.0f, height - 1.0f, 2.0f};
+    const float d[3] = {fma_(-rf[0], V4[3], V1[3]), fma_(-rf[1], V4[3], V2[3]), fma_(-rf[2], V4[3], V3[3])};
+    const bool bx = d[0] > 0.0f && d[0] < ex[0] * V4[3];
+    const bool by = d[1] > 0.0f && d[1] < ex[1] * V4[3];
+    const bool bz = d[2] > 0.0f && d[2] < ex[2] * V4[3];
+    if (bx && by && bz) return 0.0f;
+    const float dx = copysignf(ex[0] * 0.5f, fma_(-(ex[0] * 0.5f), V4[3], d[0]));
+    const float dy = copysignf(ex[1] * 0.5f, fma_(-(ex[1] * 0.5f), V4[3], d[1]));
+    const float sx = fma_(ex[0], 0.5f, rf[0]) + dx;
+    const float sy = fma_(ex[1], 0.5f, rf[1]) + dy;
+    float cpx[4], cpy[4];
+    for (int c = 0; c < 4; ++c) {
+        cpx[c] = fma_(-V4[c], sx, V1[c]);
+        cpy[c] = fma_(-V4[c], sy, V2[c]);
+    }
+    float ps[4];
+    float c = aaa_contrib_plane(cpx, V1, V2, V3, V4, ps);
+    if (c < best && aaa_in_range(ps[1], rf[1], ex[1], ps[3]) && aaa_in_range(ps[2], rf[2], ex[2], ps[3])) best = c;
+    c = aaa_contrib_plane(cpy, V1, V2, V3, V4, ps);
+    if (c < best && aaa_in_range(ps[0], rf[0], ex[0], ps[3]) && aaa_in_range(ps[2], rf[2], ex[2], ps[3])) best = c;
+    c = aaa_contrib_ray(cpx, cpy, V1, V2, V3, V4, ps);
+    if (c < best && aaa_in_range(ps[2], rf[2], ex[2], ps[3])) best = c;
+    const float oy = fma_(ex[1], 0.5f, rf[1]) - dy;
+    float opy[4];
+    for (int k = 0; k < 4; ++k) opy[k] = fma_(-V4[k], oy, V2[k]);
+    c = aaa_contrib_ray(cpx, opy, V1, V2, V3, V4, ps);
+    if (c < best && aaa_in_range(ps[2], rf[2], ex[2], ps[3])) best = c;
+    const float ox = fma_(ex[0], 0.5f, rf[0]) - dx;
+    float opx[4];
+    for (int k = 0; k < 4; ++k) opx[k] = fma_(-V4[k], ox, V1[k]);
+    c = aaa_contrib_ray(opx, cpy, V1, V2, V3, V4, ps);
+    if (c < best && aaa_in_range(ps[2], rf[2], ex[2], ps[3])) best = c;
+    return best;
+}
+
+// kernel_utils_aaa.cuh:135-144 (fmodf is exact in IEEE arithmetic)
+inline float aaa_normalize_angle(float th) {
+    const float pi = 3.141592654f;
+    const float two_pi = 2.0f * pi;
+    th = fmodf(th, two_pi);
+    if (th > pi) th -= two_pi;
+    if (th <= -pi) th += two_pi;
+    return th;
+}
+
+// kernel_utils_aaa.cuh:162-201 for one axis; returns (tan lo, tan hi)
+void aaa_theta(const float* viewdir, const float* MTa, const float* MT3, float cutoff, bool is_x, float* out) {
+    const float pi = 3.141592654f;
+    const float two_pi = 2.0f * pi;
+    const float max_theta = fma_(0.5f, pi, -1e-5f);
+    const float t[4] = {cutoff, cutoff, cutoff, -1.0f};
+    const float mu = fh_atan2f(is_x ? viewdir[0] : viewdir[1], viewdir[2]);
+    float qa[4], qz[4], qm[4];
+    for (int c = 0; c < 4; ++c) { qa[c] = MTa[c] * MTa[c]; qz[c] = MT3[c] * MT3[c]; qm[c] = MTa[c] * MT3[c]; }
+    const float sa = dot4(t, qa), sz = dot4(t, qz), mid = dot4(t, qm);
+    const float inside = fma_(mid, mid, -(sz * sa));
+    float r0 = -max_theta, r1 = max_theta;
+    if (inside > 0.0f) {
+        const float sq = sqrtf(inside);
+        float a0 = fh_atan2f(-(mid + sq), -sz);
+        float a1 = fh_atan2f(-(mid - sq), -sz);
+        while (a0 > mu) a0 -= pi;
+        while (a0 < (mu - pi)) a0 += pi;
+        while (a1 < mu) a1 += pi;
+        while (a1 > (mu + pi)) a1 -= pi;
+        const float n0 = aaa_normalize_angle(a0), n1 = aaa_normalize_angle(a1);
+        if (mu < 0.0f && fabsf(n0) < fabsf(n1)) { a0 += two_pi; a1 += two_pi; }
+        else if (mu > 0.0f && fabsf(n1) < fabsf(n0)) { a0 -= two_pi; a1 -= two_pi; }
+        r0 = fh_fmax(r0, a0);
+        r1 = fh_fmin(r1, a1);
+    }
+    out[0] = fh_tanf(r0);
+    out[1] = fh_tanf(r1);
+}
+
+// rasterization/include/utils/kernel_utils_aaa.cuh:210-363. Returns true when
+// culled; `opacity` comes back dilated.
+bool transform_and_cull_aaa(const orc_scene& sc, const orc_view& v, int i, int gw, int gh,
+                            const Fov& fov, int gtx, int gty, uint32_t& n_touched, uint32_t* b,
+                            float* V1, float* V2, float* V4, float& opacity) {
+    const float* mu = sc.positions + 3 * (int64_t)i;
+    const float* os = sc.scales + 3 * (int64_t)i;
+    const float* q = sc.rotations + 4 * (int64_t)i;
+    const float r = q[0], x = q[1], y = q[2], zq = q[3];
+    const float xx = x * x, yy = y * y, zz = zq * zq;
+    const float xy = x * y, xz = x * zq, yz = y * zq;
+    const float rx = r * x, ry = r * y, rz = r * zq;
+    const float R11 = fma_(-2.0f, yy + zz, 1.0f), R12 = 2.0f * (xy - rz), R13 = 2.0f * (xz + ry);
+    const float R21 = 2.0f * (xy + rz), R22 = fma_(-2.0f, xx + zz, 1.0f), R23 = 2.0f * (yz - rx);
+    const float R31 = 2.0f * (xz - ry), R32 = 2.0f * (yz + rx), R33 = fma_(-2.0f, xx + yy, 1.0f);
+    const float* M3 = v.M + 8;
+    const float z = dot3(M3, mu) + M3[3];
+    const float max_focal = fh_fmax(v.focal_x, v.focal_y);
+    const float fsv = z / max_focal;
+    const float fvv = (fsv * fsv) * 0.3f;
+    const float ov[3] = {os[0] * os[0], os[1] * os[1], os[2] * os[2]};
+    const float var[3] = {ov[0] + fvv, ov[1] + fvv, ov[2] + fvv};
+    const float s[3] = {sqrtf(var[0]), sqrtf(var[1]), sqrtf(var[2])};
+    const float dw0[3] = {mu[0] - v.cam_position[0], mu[1] - v.cam_position[1], mu[2] - v.cam_position[2]};
+    float dw[3];
+    normalize3(dw0, dw);
+    const float dg[3] = {fma_(R31, dw[2], fma_(R21, dw[1], R11 * dw[0])),
+                         fma_(R32, dw[2], fma_(R22, dw[1], R12 * dw[0])),
+                         fma_(R33, dw[2], fma_(R23, dw[1], R13 * dw[0]))};
+    const float rr[3] = {dg[0] * dg[0], dg[1] * dg[1], dg[2] * dg[2]};
+    const float p0[3] = {ov[1] * ov[2], ov[2] * ov[0], ov[0] * ov[1]};
+    const float p1[3] = {var[1] * var[2], var[2] * var[0], var[0] * var[1]};
+    const float det0 = dot3(rr, p0), det1 = dot3(rr, p1);
+    opacity = opacity * sqrtf(fh_saturate(det0 / det1));
+    if (opacity < kMinAlpha) return true;
+    const float rho = 2.0f * fh_logf(opacity * kMinAlphaRcp);
+    const float cs[3] = {v.cam_position[0] - mu[0], v.cam_position[1] - mu[1], v.cam_position[2] - mu[2]};
+    const float cg[3] = {fma_(R31, cs[2], fma_(R21, cs[1], R11 * cs[0])) / s[0],
+                         fma_(R32, cs[2], fma_(R22, cs[1], R12 * cs[0])) / s[1],
+                         fma_(R33, cs[2], fma_(R23, cs[1], R13 * cs[0])) / s[2]};
+    if (dot3(cg, cg) < rho) return true;
+    const float sm = v.scale_modifier;
+    const float u[3] = {(R11 * s[0]) * sm, (R21 * s[0]) * sm, (R31 * s[0]) * sm};
+    const float vv[3] = {(R12 * s[1]) * sm, (R22 * s[1]) * sm, (R32 * s[1]) * sm};
+    const float w[3] = {(R13 * s[2]) * sm, (R23 * s[2]) * sm, (R33 * s[2]) * sm};
+    float V3[4];
+    frame_row(v.VPM + 0, u, vv, w, mu, V1);
+    frame_row(v.VPM + 4, u, vv, w, mu, V2);
+    frame_row(v.VPM + 8, u, vv, w, mu, V3);
+    frame_row(v.VPM + 12, u, vv, w, mu, V4);
+    if (aaa_contrib_frustum(V1, V2, V3, V4, (float)v.width, (float)v.height) > rho) return true;
+    float MT1[4], MT2[4];
+    frame_row(v.M + 0, u, vv, w, mu, MT1);
+    frame_row(v.M + 4, u, vv, w, mu, MT2);
+    // compute_aabb_view, kernel_utils_aaa.cuh:146-208
+    const float vd0[3] = {MT1[3], MT2[3], V4[3]};
+    float vd[3];
+    normalize3(vd0, vd);
+    float tx[2], ty[2];
+    aaa_theta(vd, MT1, V4, rho, true, tx);
+    aaa_theta(vd, MT2, V4, rho, false, ty);
+    const float bx0 = fma_(v.focal_x, tx[0], v.center_x), bx1 = fma_(v.focal_x, tx[1], v.center_x);
+    const float by0 = fma_(v.focal_y, ty[0], v.center_y), by1 = fma_(v.focal_y, ty[1], v.center_y);
+    const float ccx = (bx1 + bx0) / 2.0f, ccy = (by1 + by0) / 2.0f;
+    const float ecx = (bx1 - bx0) / 2.0f, ecy = (by1 - by0) / 2.0f;
+    // kernel_utils_aaa.cuh:311-316: no half-pixel shift here
+    auto lo = [](float c, float e, int grid) -> uint32_t {
+        const int t = std::max(0, fh_f2i_rd((c - e) / 16.0f));
+        return std::min((uint32_t)grid, (uint32_t)t);
+    };
+    auto hi = [](float c, float e, int grid) -> uint32_t {
+        const int t = std::max(0, fh_f2i_ru((c + e) / 16.0f));
+        return std::min((uint32_t)grid, (uint32_t)t);
+    };
+    b[0] = lo(ccx, ecx, gw);
+    b[1] = hi(ccx, ecx, gw);
+    b[2] = lo(ccy, ecy, gh);
+    b[3] = hi(ccy, ecy, gh);
+    const int fd = 2 * fov.R;
+    const int tlx = gtx - fov.R, tly = gty - fov.R;
+    auto fclamp = [fd](int t) -> uint32_t { return (uint32_t)std::min(fd, std::max(0, t)); };
+    const uint32_t fb0 = fclamp((int)b[0] - tlx), fb1 = fclamp((int)b[1] - tlx);
+    const uint32_t fb2 = fclamp((int)b[2] - tly), fb3 = fclamp((int)b[3] - tly);
+    const uint32_t W1 = (uint32_t)gw + 1u;
+    const uint32_t* vs = v.visibility_mask_area_table;
+    uint32_t n = vs[b[1] + W1 * b[3]] + vs[b[0] + W1 * b[2]] - vs[b[1] + W1 * b[2]] - vs[b[0] + W1 * b[3]];
+    if (fb1 != fb0 && fb3 != fb2) {
+        const uint32_t F1 = (uint32_t)fd + 1u;
+        const uint32_t* fs = v.fovea_mask_area_table;
+        n += fs[fb1 + F1 * fb3] + fs[fb0 + F1 * fb2] - fs[fb1 + F1 * fb2] - fs[fb0 + F1 * fb3];
+    }
+    n_touched = n;
+    return n == 0;
+}
+
 // rasterization/include/kernels/background_model.cuh:9-48 (sky SH, degree 3)
 void eval_sh_background(const orc_view& v, float px, float py, float* out) {
     const float pc[4] = {px, py, 1.0f, 1.0f};
@@ -551,7 +764,6 @@ orc_result* orc_run(const orc_scene* scp, const orc_view* vp, int32_t stop_after
     const orc_view& v = *vp;
     if (v.width <= 0 || v.height <= 0) return fail("width and height must be positive");
     if (sc.n_primitives < 0) return fail("negative primitive count");
-    if (v.anti_aliasing) return fail("anti-aliased preprocess is not restated in this oracle");
 
     orc_result* R = new orc_result();
     const int N = sc.n_primitives;
@@ -630,7 +842,12 @@ orc_result* orc_run(const orc_scene* scp, const orc_view* vp, int32_t stop_after
     for (int i = 0; i < N; ++i) {
         uint32_t nt = 0, b[4];
         float V1[4], V2[4], V4[4];
-        if (transform_and_cull(sc, v, i, gw, gh, fov, gtx, gty, nt, b, V1, V2, V4)) continue;
+        float opacity = sc.opacities[i];
+        // inference.cuh:86-112: the AAA variant when anti_aliasing is set
+        const bool culled = v.anti_aliasing
+                                ? transform_and_cull_aaa(sc, v, i, gw, gh, fov, gtx, gty, nt, b, V1, V2, V4, opacity)
+                                : transform_and_cull(sc, v, i, gw, gh, fov, gtx, gty, nt, b, V1, V2, V4);
+        if (culled) continue;
         ++n_visible;
         R->n_touched[i] = nt;
         for (int c = 0; c < 4; ++c) {
@@ -644,7 +861,7 @@ orc_result* orc_run(const orc_scene* scp, const orc_view* vp, int32_t stop_after
         R->rgba[4 * (size_t)i + 0] = rgb[0];
         R->rgba[4 * (size_t)i + 1] = rgb[1];
         R->rgba[4 * (size_t)i + 2] = rgb[2];
-        R->rgba[4 * (size_t)i + 3] = sc.opacities[i];
+        R->rgba[4 * (size_t)i + 3] = opacity;
     }
     double t2 = now_ms();
     R->timings[1] = t2 - t1;

@@ -39,6 +39,9 @@ CASES = [
     Case("bg_tex_raw", 15000, 320, 240, seed=20, background=3, foveation_radius_px=120, blend_width_px=50),
     Case("scale_mod", 20000, 320, 240, seed=21, scale_modifier=1.7),
     Case("empty_scene", 0, 128, 96, seed=22),
+    Case("aaa_fov", 30000, 640, 480, seed=41, aaa=True),
+    Case("aaa_blur_chw", 30000, 480, 352, seed=42, aaa=True, blur=True, to_chw=True),
+    Case("aaa_nofov_k16", 20000, 320, 240, seed=43, aaa=True, K=16, foveation_radius_px=-1),
 ]
 
 
@@ -49,7 +52,7 @@ def _gpu_render(case: Case, sc, cam, gaze, visible, bg, benchmark=False):
     st = case.settings(cam, gaze, visible, bg)
     args = S.scene_to_torch(sc, "cuda")
     fn = r.benchmark if benchmark else r.render
-    img = fn(*args, st, case.to_chw, case.blur, False)
+    img = fn(*args, st, case.to_chw, case.blur, case.aaa)
     torch.cuda.synchronize()
     return img.cpu().numpy()
 
