@@ -95,6 +95,27 @@ FH_HD float fh_expf(float x) {
     return fh_ldexp_small(p, (int32_t)k);
 }
 
+// fh_expf for x in [-64, 0] or NaN, without the range checks: the same
+// operations, and the final 2^k scaling (exact in both forms) done in one
+// multiply, so the result is bit-identical to fh_expf there. The blend's
+// exponent -rho^2/2 is >= -5.55 (rho^2 <= 11.0825) or NaN.
+FH_HD float fh_expf_nonpos(float x) {
+    if (x < -64.0f) return fh_expf(x);
+    const float k = rintf(x * 0x1.715476p+0f);
+    float r = fh_fma(-k, 0x1.62e400p-1f, x);
+    r = fh_fma(-k, 0x1.7f7d1cp-20f, r);
+    float p = 0x1.a01a02p-13f;
+    p = fh_fma(p, r, 0x1.6c16c2p-10f);
+    p = fh_fma(p, r, 0x1.111112p-7f);
+    p = fh_fma(p, r, 0x1.555556p-5f);
+    p = fh_fma(p, r, 0x1.555556p-3f);
+    p = fh_fma(p, r, 0.5f);
+    p = fh_fma(p, r, 1.0f);
+    p = fh_fma(p, r, 1.0f);
+    const int32_t ki = (k == k) ? (int32_t)k : 0;
+    return p * fh_pow2i(ki);
+}
+
 FH_HD float fh_logf(float x) {
     if (!(x == x)) return x;
     if (x < 0.0f) return fh_as_f32(0x7fc00000u);

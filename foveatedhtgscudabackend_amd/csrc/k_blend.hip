@@ -119,6 +119,31 @@ struct BlendArgs {
     const float4* rgba;
 };
 
+// Conservative test that a Gaussian can reach alpha >= 1/255 somewhere in the
+// pixel rectangle [x0, x1] x [y0, y1]: the screen-space box of the quadric
+// rho^2 <= 2 ln(255 opacity) (the support of alpha >= 1/255), grown by one
+// pixel. Only a proper ellipse (the quadric entirely in front of the camera,
+// s < 0) is culled; anything else is kept. Skipping a Gaussian that reaches
+// no pixel of the tile changes nothing, so the result stays identical to
+// evaluating every instance (inference.cuh:202-238).
+__device__ __forceinline__ bool may_touch(float4 V1, float4 V2, float4 V4, float opacity, float x0,
+                                          float x1, float y0, float y1) {
+    const float rho = 2.0f * fh_logf(opacity * kMinAlphaRcp);
+    const v4 a4 = ld4(V4);
+    const v4 d{rho, rho, rho, -1.0f};
+    const float s = dot4(d, mul4(a4, a4));
+    if (!(s < 0.0f) || !(a4.w > 0.0f)) return true;
+    const float rs = 1.0f / s;
+    const v4 f{rs * d.x, rs * d.y, rs * d.z, rs * d.w};
+    const v4 a1 = ld4(V1), a2 = ld4(V2);
+    const float cx = dot4(f, mul4(a1, a4));
+    const float ex = sqrtf(fmaxf(fmaf(cx, cx, -dot4(f, mul4(a1, a1))), 0.0f));
+    const float cy = dot4(f, mul4(a2, a4));
+    const float ey = sqrtf(fmaxf(fmaf(cy, cy, -dot4(f, mul4(a2, a2))), 0.0f));
+    const float m = 1.0f;
+    return !((cx + ex + m < x0) || (cx - ex - m > x1) || (cy + ey + m < y0) || (cy - ey - m > y1));
+}
+
 template <int K>
 __global__ __launch_bounds__(64) void blend_kernel(ViewParams vp, BlendArgs a) {
     __shared__ float4 sV1[64], sV2[64], sV4[64], sC[64];
@@ -134,10 +159,15 @@ __global__ __launch_bounds__(64) void blend_kernel(ViewParams vp, BlendArgs a) {
     const uint32_t ltx = lt % (uint32_t)vp.gw, lty = lt / (uint32_t)vp.gw;
     const uint32_t lane = threadIdx.x;
     const uint32_t step = type == kTilePeriphery ? kStride : 1u;
-    const uint32_t px = ltx * kTileL + (st % kStride) * kTileS + (lane & 7u) * step;
-    const uint32_t py = lty * kTileL + (st / kStride) * kTileS + (lane >> 3) * step;
+    const uint32_t bx = ltx * kTileL + (st % kStride) * kTileS;
+    const uint32_t by = lty * kTileL + (st / kStride) * kTileS;
+    const uint32_t px = bx + (lane & 7u) * step;
+    const uint32_t py = by + (lane >> 3) * step;
     const bool inside = px < (uint32_t)vp.width && py < (uint32_t)vp.height;
     const float pxf = (float)px, pyf = (float)py;
+    const float tx0 = (float)bx, tx1 = (float)(bx + 7u * step);
+    const float ty0 = (float)by, ty1 = (float)(by + 7u * step);
+    const uint64_t lt_mask = lanemask_lt();
 
     float cd[K];
     uint32_t cc[K];
@@ -148,24 +178,38 @@ __global__ __launch_bounds__(64) void blend_kernel(ViewParams vp, BlendArgs a) {
 
     const uint2 range = a.ranges[tg];
     for (uint32_t b0 = range.x; b0 < range.y; b0 += 64u) {
-        __syncthreads();
+        // each lane loads one instance of the next 64 and tests it against the
+        // tile; survivors are compacted in order into the LDS batch
         const uint32_t idx = b0 + lane;
+        bool keep = false;
+        float4 V1, V2, V4, C;
         if (idx < range.y) {
             const uint32_t p = a.vals[idx];
-            sV1[lane] = a.vpmt1[p];
-            sV2[lane] = a.vpmt2[p];
-            sV4[lane] = a.vpmt4[p];
-            sC[lane] = a.rgba[p];
+            V1 = a.vpmt1[p];
+            V2 = a.vpmt2[p];
+            V4 = a.vpmt4[p];
+            C = a.rgba[p];
+            keep = may_touch(V1, V2, V4, C.w, tx0, tx1, ty0, ty1);
+        }
+        const uint64_t km = __ballot(keep);
+        const int count = (int)__popcll(km);
+        if (count == 0) continue;
+        __syncthreads();
+        if (keep) {
+            const uint32_t pos = (uint32_t)__popcll(km & lt_mask);
+            sV1[pos] = V1;
+            sV2[pos] = V2;
+            sV4[pos] = V4;
+            sC[pos] = C;
         }
         __syncthreads();
         if (!inside) continue;
-        const int count = (int)min(64u, range.y - b0);
         for (int j = 0; j < count; ++j) {
-            const float4 V1 = sV1[j], V2 = sV2[j], V4 = sV4[j];
-            const float Px0 = fmaf(-V4.x, pxf, V1.x), Px1 = fmaf(-V4.y, pxf, V1.y);
-            const float Px2 = fmaf(-V4.z, pxf, V1.z), Px3 = fmaf(-V4.w, pxf, V1.w);
-            const float Py0 = fmaf(-V4.x, pyf, V2.x), Py1 = fmaf(-V4.y, pyf, V2.y);
-            const float Py2 = fmaf(-V4.z, pyf, V2.z), Py3 = fmaf(-V4.w, pyf, V2.w);
+            const float4 W1 = sV1[j], W2 = sV2[j], W4 = sV4[j];
+            const float Px0 = fmaf(-W4.x, pxf, W1.x), Px1 = fmaf(-W4.y, pxf, W1.y);
+            const float Px2 = fmaf(-W4.z, pxf, W1.z), Px3 = fmaf(-W4.w, pxf, W1.w);
+            const float Py0 = fmaf(-W4.x, pyf, W2.x), Py1 = fmaf(-W4.y, pyf, W2.y);
+            const float Py2 = fmaf(-W4.z, pyf, W2.z), Py3 = fmaf(-W4.w, pyf, W2.w);
             const v3 m{fmaf(Px3, Py0, -(Px0 * Py3)), fmaf(Px3, Py1, -(Px1 * Py3)),
                        fmaf(Px3, Py2, -(Px2 * Py3))};
             const v3 d = cross3(v3{Px0, Px1, Px2}, v3{Py0, Py1, Py2});
@@ -173,32 +217,37 @@ __global__ __launch_bounds__(64) void blend_kernel(ViewParams vp, BlendArgs a) {
             const float den = dot3(d, d);
             if (num > kMaxCutoffSq * den) continue;
             const float rcp = 1.0f / den;
-            const float G = fh_expf((-0.5f * num) * rcp);
-            const float4 C = sC[j];
-            const float alpha = fminf(C.w * G, kMaxFragmentAlpha);
+            const float G = fh_expf_nonpos((-0.5f * num) * rcp);
+            const float4 Cj = sC[j];
+            const float alpha = fminf(Cj.w * G, kMaxFragmentAlpha);
             if (alpha < kMinAlpha) continue;
-            const v3 e0 = cross3(d, m);
-            const v3 e{e0.x * rcp, e0.y * rcp, e0.z * rcp};
-            const float depth = dot3(v3{V4.x, V4.y, V4.z}, e) + V4.w;
-            float p0 = C.x * alpha, p1 = C.y * alpha, p2 = C.z * alpha, p3 = alpha;
-            if (depth < cd[K - 1] && alpha >= kMinAlphaCore) {
-                float fd = depth;
-                uint32_t fc = pack_channel(p0) | (pack_channel(p1) << 8) | (pack_channel(p2) << 16) |
-                              (pack_channel(p3) << 24);
+            float p0 = Cj.x * alpha, p1 = Cj.y * alpha, p2 = Cj.z * alpha, p3 = alpha;
+            // the core test of inference.cuh:225 with its alpha half first: the
+            // depth only matters for fragments that may enter the core
+            if (alpha >= kMinAlphaCore) {
+                const v3 e0 = cross3(d, m);
+                const float depth = fmaf(W4.z, e0.z * rcp, fmaf(W4.y, e0.y * rcp, W4.x * (e0.x * rcp))) + W4.w;
+                if (depth < cd[K - 1]) {
+                    float fd = depth;
+                    uint32_t fc = pack_channel(p0) | (pack_channel(p1) << 8) | (pack_channel(p2) << 16) |
+                                  (pack_channel(p3) << 24);
+                    // insertion into the sorted core (swap chain of inference.cuh:227-232), branch-free
 #pragma unroll
-                for (int c = 0; c < K; ++c) {
-                    if (fd < cd[c]) {
+                    for (int c = 0; c < K; ++c) {
+                        const bool lt = fd < cd[c];
                         const float td = cd[c];
                         const uint32_t tc = cc[c];
-                        cd[c] = fd; cc[c] = fc;
-                        fd = td; fc = tc;
+                        cd[c] = lt ? fd : td;
+                        cc[c] = lt ? fc : tc;
+                        fd = lt ? td : fd;
+                        fc = lt ? tc : fc;
                     }
+                    const float s = 1.0f / 255.0f;
+                    p0 = (float)(fc & 0xffu) * s;
+                    p1 = (float)((fc >> 8) & 0xffu) * s;
+                    p2 = (float)((fc >> 16) & 0xffu) * s;
+                    p3 = (float)((fc >> 24) & 0xffu) * s;
                 }
-                const float s = 1.0f / 255.0f;
-                p0 = (float)(fc & 0xffu) * s;
-                p1 = (float)((fc >> 8) & 0xffu) * s;
-                p2 = (float)((fc >> 16) & 0xffu) * s;
-                p3 = (float)((fc >> 24) & 0xffu) * s;
             }
             const float tt = 1.0f - p3;
             t0 = t0 + p0; t1 = t1 + p1; t2 = t2 + p2; t3 = t3 + p3;
@@ -212,14 +261,14 @@ __global__ __launch_bounds__(64) void blend_kernel(ViewParams vp, BlendArgs a) {
     // the loop condition of inference.cuh:246; once it fails Tc no longer changes
 #pragma unroll
     for (int c = 0; c < K; ++c) {
-        if (!(Tc >= kTransmittanceThreshold)) continue;
+        const bool on = Tc >= kTransmittanceThreshold;
         const uint32_t col = cc[c];
         const float q0 = (float)(col & 0xffu) * s255, q1 = (float)((col >> 8) & 0xffu) * s255;
         const float q2 = (float)((col >> 16) & 0xffu) * s255, q3 = (float)((col >> 24) & 0xffu) * s255;
-        r = fmaf(Tc, q0, r);
-        gch = fmaf(Tc, q1, gch);
-        bch = fmaf(Tc, q2, bch);
-        Tc = Tc * (1.0f - q3);
+        r = on ? fmaf(Tc, q0, r) : r;
+        gch = on ? fmaf(Tc, q1, gch) : gch;
+        bch = on ? fmaf(Tc, q2, bch) : bch;
+        Tc = on ? Tc * (1.0f - q3) : Tc;
     }
     if (Tc >= kTransmittanceThreshold && t3 >= kMinAlpha) {
         const float wt = Tc * (1.0f - T_tail);

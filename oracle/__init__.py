@@ -40,7 +40,7 @@ DTYPES = {
 }
 COUNT_NAMES = ("N", "n_large", "n_small", "n_active", "instances", "valid_instances", "end_bit",
                "key_bytes", "n_visible", "gw", "gh", "fovea_tiles", "blend_tiles", "gaze_tx",
-               "gaze_ty", "evaluations")
+               "gaze_ty", "evaluations", "pass_cutoff", "pass_alpha", "core_inserts", "reserved")
 STAGES = ("tiles", "preprocess", "scan", "instances", "sort", "ranges", "blend", "upsample")
 
 

@@ -150,7 +150,7 @@ inline void unpack_rgba(uint32_t c, float* o) {
 }  // namespace
 
 struct orc_result {
-    std::vector<int64_t> counts = std::vector<int64_t>(16, 0);
+    std::vector<int64_t> counts = std::vector<int64_t>(20, 0);
     std::vector<uint32_t> tile_num, tile_map, tile_map_part, partition = std::vector<uint32_t>(6, 0);
     std::vector<uint8_t> tile_type, tile_type_part;
     std::vector<uint32_t> n_touched, offsets;
@@ -956,7 +956,7 @@ orc_result* orc_run(const orc_scene* scp, const orc_view* vp, int32_t stop_after
     float* image = v.blur_periphery ? R->image_temp.data() : R->image.data();
     const int K = effective_k(v.K);
     const bool chw = v.to_chw != 0;
-    int64_t evaluations = 0;
+    int64_t evaluations = 0, pass_cutoff = 0, pass_alpha = 0, core_inserts = 0;
     std::vector<Fragment> core(K);
     for (int type = 0; type < 3; ++type) {
         const uint32_t px0 = R->partition[2 * type], px1 = R->partition[2 * type + 1];
@@ -995,16 +995,19 @@ orc_result* orc_run(const orc_scene* scp, const orc_view* vp, int32_t stop_after
                         const float num = dot3(m, m);
                         const float den = dot3(d, d);
                         if (num > kMaxCutoffSq * den) continue;
+                        ++pass_cutoff;
                         const float rcp = 1.0f / den;
                         const float G = fh_expf((-0.5f * num) * rcp);
                         const float alpha = fh_fmin(col[3] * G, kMaxFragmentAlpha);
                         if (alpha < kMinAlpha) continue;
+                        ++pass_alpha;
                         float e[3];
                         cross3(d, m, e);
                         for (int c = 0; c < 3; ++c) e[c] = e[c] * rcp;
                         const float depth = dot3(V4, e) + V4[3];
                         float pm[4] = {col[0] * alpha, col[1] * alpha, col[2] * alpha, alpha};
                         if (depth < core[K - 1].depth && alpha >= kMinAlphaCore) {
+                            ++core_inserts;
                             Fragment f{depth, pack_rgba(pm)};
                             for (int c = 0; c < K; ++c)
                                 if (f.depth < core[c].depth) std::swap(f, core[c]);
@@ -1042,6 +1045,9 @@ orc_result* orc_run(const orc_scene* scp, const orc_view* vp, int32_t stop_after
         }
     }
     R->counts[ORC_C_EVALUATIONS] = evaluations;
+    R->counts[16] = pass_cutoff;
+    R->counts[17] = pass_alpha;
+    R->counts[18] = core_inserts;
     double t7 = now_ms();
     R->timings[6] = t7 - t6;
 

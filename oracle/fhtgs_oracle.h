@@ -60,7 +60,7 @@ typedef struct {
 
 /* Result arrays, fetched by id after orc_run. */
 enum {
-    ORC_COUNTS = 0,          /* int64[16], see ORC_C_* */
+    ORC_COUNTS = 0,          /* int64[20], see ORC_C_* */
     ORC_TILE_NUM = 1,        /* u32[n_large]  0/1/4 per large tile */
     ORC_TILE_MAP = 2,        /* u32[n_active] compacted small-tile ids */
     ORC_TILE_TYPE = 3,       /* u8 [n_active] */
@@ -90,7 +90,8 @@ enum {
     ORC_C_INSTANCES = 4, ORC_C_VALID_INSTANCES = 5, ORC_C_END_BIT = 6,
     ORC_C_KEY_BYTES = 7, ORC_C_N_VISIBLE = 8, ORC_C_GRID_W = 9, ORC_C_GRID_H = 10,
     ORC_C_FOVEA_R_TILES = 11, ORC_C_BLEND_R_TILES = 12, ORC_C_GAZE_TX = 13,
-    ORC_C_GAZE_TY = 14, ORC_C_EVALUATIONS = 15
+    ORC_C_GAZE_TY = 14, ORC_C_EVALUATIONS = 15, ORC_C_PASS_CUTOFF = 16, ORC_C_PASS_ALPHA = 17,
+    ORC_C_CORE_INSERTS = 18
 };
 
 typedef struct orc_result orc_result;
