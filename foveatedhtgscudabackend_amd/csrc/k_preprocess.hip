@@ -335,19 +335,23 @@ __global__ __launch_bounds__(kPrimBlock) void preprocess_aaa_kernel(PreprocessAr
 
 template <int NV>
 __global__ __launch_bounds__(kPrimBlock) void preprocess_kernel(PreprocessArgs a) {
+    extern __shared__ float s_sh[];  // per wave: 64 slots of the first min(3S, 45) SH floats
     __shared__ uint32_t s_sum[NV][kPrimBlock / 64];
     __shared__ uint32_t s_vis[NV][kPrimBlock / 64];
     const uint32_t i = blockIdx.x * kPrimBlock + threadIdx.x;
     const bool valid = i < a.n;
+    const uint32_t lane = lane_id(), wid = threadIdx.x >> 6;
 
     uint32_t touched[NV];
     bool visible[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) { touched[v] = 0u; visible[v] = false; }
+    v3 mu{0.0f, 0.0f, 0.0f};
+    float opacity = 0.0f;
 
     if (valid) {
-        const v3 mu{a.pos[3 * i + 0], a.pos[3 * i + 1], a.pos[3 * i + 2]};
-        const float opacity = a.opa[i];
+        mu = v3{a.pos[3 * i + 0], a.pos[3 * i + 1], a.pos[3 * i + 2]};
+        opacity = a.opa[i];
         // view depth cull first (kernel_utils.cuh:121-123)
         bool any = false;
         bool zok[NV];
@@ -396,21 +400,7 @@ __global__ __launch_bounds__(kPrimBlock) void preprocess_kernel(PreprocessArgs a
                 const float ey = sqrtf(fmaxf(fmaf(cy, cy, -dot4(f, mul4(V2, V2))), 0.0f));
                 const uint32_t b0 = tile_lo(cx, ex, vp.gw), b1 = tile_hi(cx, ex, vp.gw);
                 const uint32_t b2 = tile_lo(cy, ey, vp.gh), b3 = tile_hi(cy, ey, vp.gh);
-                // kernel_utils.cuh:168-210
-                const int fd = 2 * vp.fovea_r;
-                const int tlx = vp.gtx - vp.fovea_r, tly = vp.gty - vp.fovea_r;
-                const uint32_t f0 = (uint32_t)min(fd, max(0, (int)b0 - tlx));
-                const uint32_t f1 = (uint32_t)min(fd, max(0, (int)b1 - tlx));
-                const uint32_t f2 = (uint32_t)min(fd, max(0, (int)b2 - tly));
-                const uint32_t f3 = (uint32_t)min(fd, max(0, (int)b3 - tly));
-                const uint32_t W1 = (uint32_t)vp.gw + 1u;
-                const uint32_t* vs = vp.vis_sat;
-                uint32_t nt = vs[b1 + W1 * b3] + vs[b0 + W1 * b2] - vs[b1 + W1 * b2] - vs[b0 + W1 * b3];
-                if (f1 != f0 && f3 != f2) {
-                    const uint32_t F1 = (uint32_t)fd + 1u;
-                    const uint32_t* fs = vp.fovea_sat;
-                    nt += fs[f1 + F1 * f3] + fs[f0 + F1 * f2] - fs[f1 + F1 * f2] - fs[f0 + F1 * f3];
-                }
+                const uint32_t nt = tiles_touched(vp, b0, b1, b2, b3);
                 if (nt == 0u) continue;
                 touched[v] = nt;
                 visible[v] = true;
@@ -420,15 +410,43 @@ __global__ __launch_bounds__(kPrimBlock) void preprocess_kernel(PreprocessArgs a
                 vb.vpmt1[i] = st4(V1);
                 vb.vpmt2[i] = st4(V2);
                 vb.vpmt4[i] = st4(V4);
-                const v3 cam{vp.cam[0], vp.cam[1], vp.cam[2]};
-                const v3 rgb = sh_color(a.sh0 + 3 * (size_t)i, a.shr + (size_t)i * vp.total_sh * 3, mu,
-                                        cam, vp.active_sh, vp.total_sh);
-                vb.rgba[i] = make_float4(rgb.x, rgb.y, rgb.z, opacity);
             }
         }
     }
 
-    const uint32_t lane = lane_id(), wid = threadIdx.x >> 6;
+    // SH colour of the visible Gaussians (sh_utils.cuh:32-69). The wave first
+    // gathers each visible Gaussian's coefficients with one contiguous load
+    // across the lanes into LDS, instead of 45 scattered loads per lane.
+    bool any_vis = false;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) any_vis |= visible[v];
+    const int active = a.vp[0].active_sh;
+    const int total = a.vp[0].total_sh;
+    const int shw = min(3 * total, 45);
+    float* wsh = s_sh + (size_t)wid * 64 * shw;
+    const uint64_t need = __ballot(any_vis && active > 1);
+    const uint32_t slot = (uint32_t)__popcll(need & lanemask_lt());
+    uint32_t k = 0;
+    for (uint64_t m = need; m; m &= m - 1ull, ++k) {
+        const int src = __ffsll((unsigned long long)m) - 1;
+        const uint32_t ip = __shfl(i, src, 64);
+        if ((int)lane < shw) wsh[k * shw + lane] = a.shr[(size_t)ip * (3 * total) + lane];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (any_vis) {
+        const float* coef = wsh + slot * shw;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            if (!visible[v]) continue;
+            const ViewParams& vp = a.vp[v];
+            const v3 cam{vp.cam[0], vp.cam[1], vp.cam[2]};
+            const v3 rgb = sh_color(a.sh0 + 3 * (size_t)i, coef, mu, cam, vp.active_sh, vp.total_sh);
+            a.vb[v].rgba[i] = make_float4(rgb.x, rgb.y, rgb.z, opacity);
+        }
+    }
+
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
         if (valid) a.vb[v].n_touched[i] = touched[v];
@@ -461,8 +479,9 @@ hipError_t launch_preprocess(const ViewParams* vps, const ViewBuffers* vbs, int 
         if (n_views == 1) preprocess_aaa_kernel<1><<<blocks, kPrimBlock, 0, s>>>(a);
         else preprocess_aaa_kernel<2><<<blocks, kPrimBlock, 0, s>>>(a);
     } else {
-        if (n_views == 1) preprocess_kernel<1><<<blocks, kPrimBlock, 0, s>>>(a);
-        else preprocess_kernel<2><<<blocks, kPrimBlock, 0, s>>>(a);
+        const size_t lds = sizeof(float) * (size_t)kPrimBlock * (size_t)min(3 * vps[0].total_sh, 45);
+        if (n_views == 1) preprocess_kernel<1><<<blocks, kPrimBlock, lds, s>>>(a);
+        else preprocess_kernel<2><<<blocks, kPrimBlock, lds, s>>>(a);
     }
     return hipGetLastError();
 }
