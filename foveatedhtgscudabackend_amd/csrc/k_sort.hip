@@ -357,23 +357,38 @@ __global__ __launch_bounds__(kSortThreads) void onesweep_kernel(PassArgs a) {
     for (int d = tid; d < bins; d += kSortThreads) {
         uint32_t excl = 0;
         if (tile != 0) {
+            // windowed look-back: four predecessor words in flight per step,
+            // consumed nearest first until an inclusive prefix is found
             int j = (int)tile - 1;
             uint32_t spins = 0;
-            while (j >= 0) {
-                const uint32_t sv = atomic_load_agent(&a.status[(size_t)j * bins + d]);
-                const uint32_t flag = sv & ~kValueMask;
-                if (flag == 0u) {
+            bool done = false;
+            while (!done && j >= 0) {
+                uint32_t sv[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    sv[q] = (j - q >= 0) ? atomic_load_agent(&a.status[(size_t)(j - q) * bins + d]) : kFlagInclusive;
+                bool stalled = false;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (done || stalled) continue;
+                    const uint32_t flag = sv[q] & ~kValueMask;
+                    if (flag == 0u) {
+                        stalled = true;
+                        continue;
+                    }
+                    excl += sv[q] & kValueMask;
+                    --j;
+                    if (flag == kFlagInclusive) done = true;
+                }
+                if (stalled) {
                     if (++spins >= kMaxSpin) {
                         atomicOr(a.error, kSortErrorBit);
                         break;
                     }
                     __builtin_amdgcn_s_sleep(1);
-                    continue;
+                } else {
+                    spins = 0;
                 }
-                spins = 0;
-                excl += sv & kValueMask;
-                if (flag == kFlagInclusive) break;
-                --j;
             }
             atomic_store_agent(&st[d], kFlagInclusive | (excl + s_tot[d]));
         }
