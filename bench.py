@@ -83,6 +83,8 @@ def main() -> int:
     ap.add_argument("--cpu-baseline", dest="cpu_baseline", action="store_true", default=True)
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
     ap.add_argument("--cpu-tile-stride", type=int, default=4)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run 3 plain device copies of 256 MiB (calibrates FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -132,6 +134,14 @@ def main() -> int:
         torch.cuda.synchronize()
         total = time.perf_counter() - t_start
         return total, lat, stages
+
+    if args.copy_probe:
+        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     total, lat, stages = run(stereo, args.steps, args.warmup, True)
     counts_eye = [handle.counts(0), handle.counts(1)]

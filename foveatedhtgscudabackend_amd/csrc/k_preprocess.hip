@@ -13,6 +13,7 @@
 namespace fhtgs {
 
 constexpr int kMaxViewsPerPass = 2;
+constexpr int kShChunk = 8;  // visible Gaussians per SH gather round
 
 struct PreprocessArgs {
     ViewParams vp[kMaxViewsPerPass];
@@ -414,35 +415,69 @@ __global__ __launch_bounds__(kPrimBlock) void preprocess_kernel(PreprocessArgs a
         }
     }
 
-    // SH colour of the visible Gaussians (sh_utils.cuh:32-69). The wave first
-    // gathers each visible Gaussian's coefficients with one contiguous load
-    // across the lanes into LDS, instead of 45 scattered loads per lane.
+    // SH colour of the visible Gaussians (sh_utils.cuh:32-69). The wave
+    // gathers the coefficients of up to 8 visible Gaussians at a time, each
+    // with one contiguous load across the lanes (8 loads in flight), stages
+    // them in LDS, and the owning lanes evaluate -- instead of 45 scattered
+    // loads per lane.
     bool any_vis = false;
 #pragma unroll
     for (int v = 0; v < NV; ++v) any_vis |= visible[v];
     const int active = a.vp[0].active_sh;
     const int total = a.vp[0].total_sh;
     const int shw = min(3 * total, 45);
-    float* wsh = s_sh + (size_t)wid * 64 * shw;
-    const uint64_t need = __ballot(any_vis && active > 1);
-    const uint32_t slot = (uint32_t)__popcll(need & lanemask_lt());
-    uint32_t k = 0;
-    for (uint64_t m = need; m; m &= m - 1ull, ++k) {
-        const int src = __ffsll((unsigned long long)m) - 1;
-        const uint32_t ip = __shfl(i, src, 64);
-        if ((int)lane < shw) wsh[k * shw + lane] = a.shr[(size_t)ip * (3 * total) + lane];
+    float* wsh = s_sh + (size_t)wid * kShChunk * shw;
+    uint64_t need = __ballot(any_vis && active > 1);
+    const uint64_t lt = lanemask_lt();
+    while (need) {
+        uint64_t chunk = 0;
+        int src[kShChunk];
+#pragma unroll
+        for (int q = 0; q < kShChunk; ++q) {
+            src[q] = need ? __ffsll((unsigned long long)need) - 1 : -1;
+            if (need) {
+                chunk |= 1ull << src[q];
+                need &= need - 1ull;
+            }
+        }
+        float c[kShChunk];
+#pragma unroll
+        for (int q = 0; q < kShChunk; ++q) {
+            c[q] = 0.0f;
+            if (src[q] >= 0) {
+                const uint32_t ip = __shfl(i, src[q], 64);
+                if ((int)lane < shw) c[q] = a.shr[(size_t)ip * (3 * total) + lane];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kShChunk; ++q)
+            if (src[q] >= 0 && (int)lane < shw) wsh[q * shw + lane] = c[q];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if ((chunk >> lane) & 1ull) {
+            const float* coef = wsh + (uint32_t)__popcll(chunk & lt) * shw;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                if (!visible[v]) continue;
+                const ViewParams& vp = a.vp[v];
+                const v3 cam{vp.cam[0], vp.cam[1], vp.cam[2]};
+                const v3 rgb = sh_color(a.sh0 + 3 * (size_t)i, coef, mu, cam, vp.active_sh, vp.total_sh);
+                a.vb[v].rgba[i] = make_float4(rgb.x, rgb.y, rgb.z, opacity);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    if (any_vis) {
-        const float* coef = wsh + slot * shw;
+    // degree 0 needs no gather
+    if (any_vis && active <= 1) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             if (!visible[v]) continue;
             const ViewParams& vp = a.vp[v];
             const v3 cam{vp.cam[0], vp.cam[1], vp.cam[2]};
-            const v3 rgb = sh_color(a.sh0 + 3 * (size_t)i, coef, mu, cam, vp.active_sh, vp.total_sh);
+            const v3 rgb = sh_color(a.sh0 + 3 * (size_t)i, a.shr, mu, cam, vp.active_sh, vp.total_sh);
             a.vb[v].rgba[i] = make_float4(rgb.x, rgb.y, rgb.z, opacity);
         }
     }
@@ -479,7 +514,7 @@ hipError_t launch_preprocess(const ViewParams* vps, const ViewBuffers* vbs, int 
         if (n_views == 1) preprocess_aaa_kernel<1><<<blocks, kPrimBlock, 0, s>>>(a);
         else preprocess_aaa_kernel<2><<<blocks, kPrimBlock, 0, s>>>(a);
     } else {
-        const size_t lds = sizeof(float) * (size_t)kPrimBlock * (size_t)min(3 * vps[0].total_sh, 45);
+        const size_t lds = sizeof(float) * (size_t)(kPrimBlock / 64) * kShChunk * (size_t)min(3 * vps[0].total_sh, 45);
         if (n_views == 1) preprocess_kernel<1><<<blocks, kPrimBlock, lds, s>>>(a);
         else preprocess_kernel<2><<<blocks, kPrimBlock, lds, s>>>(a);
     }
