@@ -79,6 +79,8 @@ def main() -> int:
     ap.add_argument("--cube", type=float, default=20.0)
     ap.add_argument("--K", type=int, default=8)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--xcd-run", type=int, default=None,
+                    help="blend tiles per XCD run (library default when omitted)")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-view-per-launch comparison")
     ap.add_argument("--cpu-baseline", dest="cpu_baseline", action="store_true", default=True)
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
@@ -108,6 +110,8 @@ def main() -> int:
     rast = FoveatedHTGSRasterizer()
     handle = _lib.handle_for(torch.cuda.current_device())
     handle.set_option(_lib.OPT_STAGE_TIMING, 1)
+    if args.xcd_run is not None:
+        handle.set_option(_lib.OPT_BLEND_XCD_RUN, args.xcd_run)
     torch.cuda.synchronize()
 
     def stereo():

@@ -33,6 +33,7 @@ DBG = {
 OPT_KEEP_UNSORTED = 1
 OPT_SORT_DIGIT_BITS = 2
 OPT_STAGE_TIMING = 3
+OPT_BLEND_XCD_RUN = 4
 NUM_STAGES = 11
 STAGE_NAMES = ("clear", "tiles", "preprocess", "scan", "sync", "instances", "sort", "ranges",
                "blend", "upsample", "total")

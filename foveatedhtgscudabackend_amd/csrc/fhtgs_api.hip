@@ -98,6 +98,7 @@ struct fhtgs_context {
     bool have_done = false;
     int keep_unsorted = 0;
     int digit_bits = 11;
+    int xcd_run = 16;
     int timing = 0;
     hipEvent_t ev[FHTGS_NUM_STAGES + 1] = {};
     float stage_ms[FHTGS_NUM_STAGES] = {};
@@ -162,10 +163,7 @@ void carve_fixed(void* base, int64_t N, int n_large, int n_small, ViewBuffers& v
     vb.ranges = c.take<uint2>((size_t)n_small);
     vb.n_touched = c.take<uint32_t>((size_t)N);
     vb.bounds = c.take<ushort4>((size_t)N);
-    vb.vpmt1 = c.take<float4>((size_t)N);
-    vb.vpmt2 = c.take<float4>((size_t)N);
-    vb.vpmt4 = c.take<float4>((size_t)N);
-    vb.rgba = c.take<float4>((size_t)N);
+    vb.prim = c.take<float4>((size_t)N * 4);
     vb.block_sums = c.take<uint32_t>(std::max<size_t>(nb, 1));
     bytes = align_up(c.off);
 }
@@ -334,6 +332,10 @@ int fhtgs_set_option(fhtgs_handle h, int32_t option, int64_t value) {
             h->digit_bits = (int)value;
             return FHTGS_OK;
         case FHTGS_OPT_STAGE_TIMING: h->timing = value ? 1 : 0; return FHTGS_OK;
+        case FHTGS_OPT_BLEND_XCD_RUN:
+            if (value < 0 || value > 4096) return set_err(h, FHTGS_E_INVALID, "XCD run must be 0..4096");
+            h->xcd_run = (int)value;
+            return FHTGS_OK;
         default: return set_err(h, FHTGS_E_INVALID, "unknown option");
     }
 }
@@ -490,7 +492,7 @@ int fhtgs_render(fhtgs_handle h, const fhtgs_scene* scene, const fhtgs_view* vie
         ViewSlot& slot = h->slots[v];
         if (views[v].clear_output)
             HCHK(launch_clear_image(slot.vp, slot.vb, slot.n_active <= 1 ? 1 : 0, s));
-        HCHK(launch_blend(slot.vp, slot.vb, slot.ib.vals[slot.sorted_src], views[v].K, slot.n_active, s));
+        HCHK(launch_blend(slot.vp, slot.vb, slot.ib.vals[slot.sorted_src], views[v].K, slot.n_active, h->xcd_run, s));
     }
     if ((rc = record(h, 9, s))) return rc;
     for (int v = 0; v < n_views; ++v) {
@@ -543,7 +545,7 @@ int fhtgs_debug_read(fhtgs_handle h, int32_t view, int32_t id, void* dst, int64_
     const int64_t N = sl.n_prims, I = sl.n_instances, na = sl.n_active;
     const void* src = nullptr;
     int64_t bytes = 0;
-    bool widen = false;
+    bool widen = false, strided = false;
     switch (id) {
         case FHTGS_DBG_TILE_NUM: src = sl.vb.tile_num; bytes = (int64_t)sl.vp.n_large * 4; break;
         case FHTGS_DBG_TILE_MAP: src = sl.vb.tile_map; bytes = na * 4; break;
@@ -552,10 +554,10 @@ int fhtgs_debug_read(fhtgs_handle h, int32_t view, int32_t id, void* dst, int64_
         case FHTGS_DBG_PARTITION: src = sl.vb.frame_info + FI_PART; bytes = 24; break;
         case FHTGS_DBG_N_TOUCHED: src = sl.vb.n_touched; bytes = N * 4; break;
         case FHTGS_DBG_BOUNDS: src = sl.vb.bounds; bytes = N * 8; break;
-        case FHTGS_DBG_VPMT1: src = sl.vb.vpmt1; bytes = N * 16; break;
-        case FHTGS_DBG_VPMT2: src = sl.vb.vpmt2; bytes = N * 16; break;
-        case FHTGS_DBG_VPMT4: src = sl.vb.vpmt4; bytes = N * 16; break;
-        case FHTGS_DBG_RGBA: src = sl.vb.rgba; bytes = N * 16; break;
+        case FHTGS_DBG_VPMT1: src = sl.vb.prim + 0; bytes = N * 16; strided = true; break;
+        case FHTGS_DBG_VPMT2: src = sl.vb.prim + 1; bytes = N * 16; strided = true; break;
+        case FHTGS_DBG_VPMT4: src = sl.vb.prim + 2; bytes = N * 16; strided = true; break;
+        case FHTGS_DBG_RGBA: src = sl.vb.prim + 3; bytes = N * 16; strided = true; break;
         case FHTGS_DBG_KEYS_UNSORTED:
             if (!h->keep_unsorted) return set_err(h, FHTGS_E_INVALID, "FHTGS_OPT_KEEP_UNSORTED is off");
             src = sl.dbg_keys; bytes = I * 4; break;
@@ -578,6 +580,9 @@ int fhtgs_debug_read(fhtgs_handle h, int32_t view, int32_t id, void* dst, int64_
             HCHK(hipMemcpy(tmp.data(), src, (size_t)I * 2, hipMemcpyDeviceToHost));
             uint32_t* o = (uint32_t*)dst;
             for (int64_t k = 0; k < I; ++k) o[k] = tmp[(size_t)k];
+        } else if (strided) {
+            // one float4 field out of each 64-B primitive record
+            HCHK(hipMemcpy2D(dst, 16, src, 64, 16, (size_t)N, hipMemcpyDeviceToHost));
         } else {
             HCHK(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost));
         }

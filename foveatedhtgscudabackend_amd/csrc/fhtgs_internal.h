@@ -81,10 +81,7 @@ struct ViewBuffers {
     uint2* ranges;            // n_small
     uint32_t* n_touched;      // N
     ushort4* bounds;          // N
-    float4* vpmt1;            // N
-    float4* vpmt2;            // N
-    float4* vpmt4;            // N
-    float4* rgba;             // N
+    float4* prim;             // 4N: one 64-B record per primitive (VPMT1, VPMT2, VPMT4, rgba)
     uint32_t* block_sums;     // ceil(N / kPrimBlock), then exclusive offsets
 };
 
@@ -127,7 +124,7 @@ hipError_t launch_sort_histogram(const InstanceBuffers& ib, const SortPlan& plan
 hipError_t launch_ranges(const ViewBuffers& vb, const InstanceBuffers& ib, const SortPlan& plan,
                          int src, uint32_t n, hipStream_t s);
 hipError_t launch_blend(const ViewParams& vp, const ViewBuffers& vb, const uint32_t* sorted_vals,
-                        int K, uint32_t n_active, hipStream_t s);
+                        int K, uint32_t n_active, int xcd_run, hipStream_t s);
 hipError_t launch_upsample(const ViewParams& vp, const ViewBuffers& vb, uint32_t n_periphery,
                            uint32_t n_blended, hipStream_t s);
 hipError_t launch_clear_image(const ViewParams& vp, const ViewBuffers& vb, int clear_all,

@@ -8,6 +8,7 @@
 // registers with 8-bit quantised colour, everything else is accumulated
 // order-independently in the tail; resolve, optional background, saturate.
 #include "fhtgs_device.h"
+#include "fhtgs_cull.h"
 
 namespace fhtgs {
 
@@ -113,41 +114,29 @@ struct BlendArgs {
     const uint32_t* tile_map_part;
     const uint2* ranges;
     const uint32_t* vals;
-    const float4* vpmt1;
-    const float4* vpmt2;
-    const float4* vpmt4;
-    const float4* rgba;
+    const float4* prim;  // 4 float4 per primitive: VPMT1, VPMT2, VPMT4, rgba
+    uint32_t xcd_run;
 };
 
-// Conservative test that a Gaussian can reach alpha >= 1/255 somewhere in the
-// pixel rectangle [x0, x1] x [y0, y1]: the screen-space box of the quadric
-// rho^2 <= 2 ln(255 opacity) (the support of alpha >= 1/255), grown by one
-// pixel. Only a proper ellipse (the quadric entirely in front of the camera,
-// s < 0) is culled; anything else is kept. Skipping a Gaussian that reaches
-// no pixel of the tile changes nothing, so the result stays identical to
-// evaluating every instance (inference.cuh:202-238).
-__device__ __forceinline__ bool may_touch(float4 V1, float4 V2, float4 V4, float opacity, float x0,
-                                          float x1, float y0, float y1) {
-    const float rho = 2.0f * fh_logf(opacity * kMinAlphaRcp);
-    const v4 a4 = ld4(V4);
-    const v4 d{rho, rho, rho, -1.0f};
-    const float s = dot4(d, mul4(a4, a4));
-    if (!(s < 0.0f) || !(a4.w > 0.0f)) return true;
-    const float rs = 1.0f / s;
-    const v4 f{rs * d.x, rs * d.y, rs * d.z, rs * d.w};
-    const v4 a1 = ld4(V1), a2 = ld4(V2);
-    const float cx = dot4(f, mul4(a1, a4));
-    const float ex = sqrtf(fmaxf(fmaf(cx, cx, -dot4(f, mul4(a1, a1))), 0.0f));
-    const float cy = dot4(f, mul4(a2, a4));
-    const float ey = sqrtf(fmaxf(fmaf(cy, cy, -dot4(f, mul4(a2, a2))), 0.0f));
-    const float m = 1.0f;
-    return !((cx + ex + m < x0) || (cx - ex - m > x1) || (cy + ey + m < y0) || (cy - ey - m > y1));
+// The dispatcher hands workgroups to the 8 XCDs round-robin by index, and
+// each XCD has its own L2. Within every group of 8*run workgroups, workgroup
+// b becomes tile (b % 8) * run + b / 8 of the group, so each XCD walks runs
+// of `run` consecutive tiles (neighbours share most of their Gaussians). A
+// trailing partial group keeps its order; the map is a bijection on [0, n).
+constexpr uint32_t kNumXcd = 8;
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n, uint32_t run) {
+    if (run <= 1u) return b;
+    const uint32_t span = kNumXcd * run;
+    const uint32_t grp = b / span;
+    if ((uint64_t)(grp + 1u) * span > n) return b;
+    const uint32_t w = b - grp * span;
+    return grp * span + (w % kNumXcd) * run + w / kNumXcd;
 }
 
 template <int K>
 __global__ __launch_bounds__(64) void blend_kernel(ViewParams vp, BlendArgs a) {
     __shared__ float4 sV1[64], sV2[64], sV4[64], sC[64];
-    const uint32_t g = blockIdx.x;
+    const uint32_t g = xcd_remap(blockIdx.x, gridDim.x, a.xcd_run);
     const uint32_t* fi = a.frame_info;
     int type;
     if (g >= fi[0] && g < fi[1]) type = kTileFovea;
@@ -178,18 +167,22 @@ __global__ __launch_bounds__(64) void blend_kernel(ViewParams vp, BlendArgs a) {
 
     const uint2 range = a.ranges[tg];
     for (uint32_t b0 = range.x; b0 < range.y; b0 += 64u) {
-        // each lane loads one instance of the next 64 and tests it against the
-        // tile; survivors are compacted in order into the LDS batch
+        // each lane loads one instance of the next 64 and tests its alpha
+        // support ellipse against the tile's samples (fh_tile_may_touch);
+        // survivors are compacted in order into the LDS batch
         const uint32_t idx = b0 + lane;
         bool keep = false;
         float4 V1, V2, V4, C;
         if (idx < range.y) {
             const uint32_t p = a.vals[idx];
-            V1 = a.vpmt1[p];
-            V2 = a.vpmt2[p];
-            V4 = a.vpmt4[p];
-            C = a.rgba[p];
-            keep = may_touch(V1, V2, V4, C.w, tx0, tx1, ty0, ty1);
+            const float4* rec = a.prim + 4 * (size_t)p;
+            V1 = rec[0];
+            V2 = rec[1];
+            V4 = rec[2];
+            C = rec[3];
+            const float a1[4] = {V1.x, V1.y, V1.z, V1.w}, a2[4] = {V2.x, V2.y, V2.z, V2.w};
+            const float a4[4] = {V4.x, V4.y, V4.z, V4.w};
+            keep = fh_tile_may_touch(a1, a2, a4, C.w, tx0, tx1, ty0, ty1);
         }
         const uint64_t km = __ballot(keep);
         const int count = (int)__popcll(km);
@@ -302,17 +295,15 @@ __global__ __launch_bounds__(64) void blend_kernel(ViewParams vp, BlendArgs a) {
 }
 
 hipError_t launch_blend(const ViewParams& vp, const ViewBuffers& vb, const uint32_t* sorted_vals,
-                        int K, uint32_t n_active, hipStream_t s) {
+                        int K, uint32_t n_active, int xcd_run, hipStream_t s) {
     if (n_active == 0) return hipSuccess;
     BlendArgs a;
     a.frame_info = vb.frame_info;
     a.tile_map_part = vb.tile_map_part;
     a.ranges = vb.ranges;
     a.vals = sorted_vals;
-    a.vpmt1 = vb.vpmt1;
-    a.vpmt2 = vb.vpmt2;
-    a.vpmt4 = vb.vpmt4;
-    a.rgba = vb.rgba;
+    a.prim = vb.prim;
+    a.xcd_run = (uint32_t)(xcd_run > 0 ? xcd_run : 0);
     // rasterization/src/inference.cu:17-31: largest power of two <= K in 1..32
     if (K >= 32) blend_kernel<32><<<n_active, 64, 0, s>>>(vp, a);
     else if (K >= 16) blend_kernel<16><<<n_active, 64, 0, s>>>(vp, a);

@@ -307,13 +307,14 @@ __global__ __launch_bounds__(kPrimBlock) void preprocess_aaa_kernel(PreprocessAr
             visible[v] = true;
             const ViewBuffers& vb = a.vb[v];
             vb.bounds[i] = make_ushort4((unsigned short)b0, (unsigned short)b1, (unsigned short)b2, (unsigned short)b3);
-            vb.vpmt1[i] = st4(V1);
-            vb.vpmt2[i] = st4(V2);
-            vb.vpmt4[i] = st4(V4);
             const v3 cam{vp.cam[0], vp.cam[1], vp.cam[2]};
             const v3 rgb = sh_color(a.sh0 + 3 * (size_t)i, a.shr + (size_t)i * vp.total_sh * 3, mu, cam,
                                     vp.active_sh, vp.total_sh);
-            vb.rgba[i] = make_float4(rgb.x, rgb.y, rgb.z, opacity);
+            float4* rec = vb.prim + 4 * (size_t)i;
+            rec[0] = st4(V1);
+            rec[1] = st4(V2);
+            rec[2] = st4(V4);
+            rec[3] = make_float4(rgb.x, rgb.y, rgb.z, opacity);
         }
     }
     const uint32_t lane = lane_id(), wid = threadIdx.x >> 6;
@@ -408,9 +409,10 @@ __global__ __launch_bounds__(kPrimBlock) void preprocess_kernel(PreprocessArgs a
                 const ViewBuffers& vb = a.vb[v];
                 vb.bounds[i] = make_ushort4((unsigned short)b0, (unsigned short)b1, (unsigned short)b2,
                                             (unsigned short)b3);
-                vb.vpmt1[i] = st4(V1);
-                vb.vpmt2[i] = st4(V2);
-                vb.vpmt4[i] = st4(V4);
+                float4* rec = vb.prim + 4 * (size_t)i;
+                rec[0] = st4(V1);
+                rec[1] = st4(V2);
+                rec[2] = st4(V4);
             }
         }
     }
@@ -463,7 +465,7 @@ __global__ __launch_bounds__(kPrimBlock) void preprocess_kernel(PreprocessArgs a
                 const ViewParams& vp = a.vp[v];
                 const v3 cam{vp.cam[0], vp.cam[1], vp.cam[2]};
                 const v3 rgb = sh_color(a.sh0 + 3 * (size_t)i, coef, mu, cam, vp.active_sh, vp.total_sh);
-                a.vb[v].rgba[i] = make_float4(rgb.x, rgb.y, rgb.z, opacity);
+                a.vb[v].prim[4 * (size_t)i + 3] = make_float4(rgb.x, rgb.y, rgb.z, opacity);
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -478,7 +480,7 @@ __global__ __launch_bounds__(kPrimBlock) void preprocess_kernel(PreprocessArgs a
             const ViewParams& vp = a.vp[v];
             const v3 cam{vp.cam[0], vp.cam[1], vp.cam[2]};
             const v3 rgb = sh_color(a.sh0 + 3 * (size_t)i, a.shr, mu, cam, vp.active_sh, vp.total_sh);
-            a.vb[v].rgba[i] = make_float4(rgb.x, rgb.y, rgb.z, opacity);
+            a.vb[v].prim[4 * (size_t)i + 3] = make_float4(rgb.x, rgb.y, rgb.z, opacity);
         }
     }
 

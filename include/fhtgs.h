@@ -167,6 +167,10 @@ int fhtgs_debug_counts(fhtgs_handle h, int32_t view, int64_t* counts);
 #define FHTGS_OPT_SORT_DIGIT_BITS 2
 /* Record per-stage HIP events; fhtgs_stage_times returns the last frame's ms. */
 #define FHTGS_OPT_STAGE_TIMING 3
+/* Blend tile-to-workgroup mapping: runs of this many consecutive tiles go to
+ * one XCD (its own L2) so neighbouring tiles reuse primitive records; 0 or 1
+ * keeps the launch order. Values 0..4096, default 16. Images do not depend on it. */
+#define FHTGS_OPT_BLEND_XCD_RUN 4
 int fhtgs_set_option(fhtgs_handle h, int32_t option, int64_t value);
 
 /* ms per stage of the last timed frame, all views of the call together:

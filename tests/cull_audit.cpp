@@ -1,0 +1,90 @@
+// Audit of the blend's per-tile cull (fh_tile_may_touch): over every (tile,
+// instance) pair of a frame, a Gaussian that the cull drops must reach no
+// sample of the tile, evaluated with the blend's own float32 operations
+// (cutoff, exp, alpha >= 1/255). Input is a binary dump of one oracle frame
+// written by tests/test_cull.py; prints pair counts and the number of
+// violations (must be 0).
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+#include "../foveatedhtgscudabackend_amd/csrc/fhtgs_cull.h"
+
+static float dot3(const float* a, const float* b) { return fmaf(a[2], b[2], fmaf(a[1], b[1], a[0] * b[0])); }
+static void cross3(const float* a, const float* b, float* o) {
+    o[0] = fmaf(a[1], b[2], -(a[2] * b[1]));
+    o[1] = fmaf(a[2], b[0], -(a[0] * b[2]));
+    o[2] = fmaf(a[0], b[1], -(a[1] * b[0]));
+}
+
+// alpha >= 1/255 at pixel (px, py): inference.cuh:202-222 as the blend computes it
+static bool reaches(const float* V1, const float* V2, const float* V4, float opacity, float pxf, float pyf) {
+    float Px[4], Py[4];
+    for (int c = 0; c < 4; ++c) {
+        Px[c] = fmaf(-V4[c], pxf, V1[c]);
+        Py[c] = fmaf(-V4[c], pyf, V2[c]);
+    }
+    float m[3], d[3];
+    for (int c = 0; c < 3; ++c) m[c] = fmaf(Px[3], Py[c], -(Px[c] * Py[3]));
+    cross3(Px, Py, d);
+    const float num = dot3(m, m), den = dot3(d, d);
+    if (num > 11.0825270903f * den) return false;
+    const float G = fh_expf((-0.5f * num) * (1.0f / den));
+    return fh_fmin(opacity * G, 1.0f) >= 1.0f / 255.0f;
+}
+
+template <class T>
+static bool rd(FILE* f, std::vector<T>& v, size_t n) {
+    v.resize(n);
+    return n == 0 || fread(v.data(), sizeof(T), n, f) == n;
+}
+
+int main(int argc, char** argv) {
+    if (argc < 2) { std::fprintf(stderr, "usage: cull_audit dump.bin\n"); return 2; }
+    FILE* f = std::fopen(argv[1], "rb");
+    if (!f) { std::perror("open"); return 2; }
+    int64_t h[7];
+    if (fread(h, 8, 7, f) != 7) return 2;
+    const int64_t N = h[0], I = h[1], n_small = h[2], n_active = h[3], gw = h[4], W = h[5], H = h[6];
+    std::vector<uint32_t> part, tmp, ranges, vals;
+    std::vector<float> v1, v2, v4, rgba;
+    if (!rd(f, part, 6) || !rd(f, tmp, n_active) || !rd(f, ranges, 2 * n_small) || !rd(f, vals, I) ||
+        !rd(f, v1, 4 * N) || !rd(f, v2, 4 * N) || !rd(f, v4, 4 * N) || !rd(f, rgba, 4 * N)) {
+        std::fprintf(stderr, "short read\n");
+        return 2;
+    }
+    std::fclose(f);
+    int64_t pairs = 0, kept = 0, reach = 0, bad = 0;
+    for (int type = 0; type < 3; ++type) {
+        const int step = type == 1 ? 2 : 1;  // periphery tiles sample their large tile at stride 2
+        for (uint32_t g = part[2 * type]; g < part[2 * type + 1]; ++g) {
+            const uint32_t tg = tmp[g], lt = tg / 4, st = tg % 4;
+            const int bx = (int)(lt % gw) * 16 + (int)(st % 2) * 8;
+            const int by = (int)(lt / gw) * 16 + (int)(st / 2) * 8;
+            const float x0 = (float)bx, x1 = (float)(bx + 7 * step);
+            const float y0 = (float)by, y1 = (float)(by + 7 * step);
+            for (uint32_t j = ranges[2 * (size_t)tg]; j < ranges[2 * (size_t)tg + 1]; ++j) {
+                const uint32_t p = vals[j];
+                const float *a1 = &v1[4 * (size_t)p], *a2 = &v2[4 * (size_t)p], *a4 = &v4[4 * (size_t)p];
+                const float o = rgba[4 * (size_t)p + 3];
+                const bool keep = fh_tile_may_touch(a1, a2, a4, o, x0, x1, y0, y1);
+                bool any = false;
+                for (int q = 0; q < 64 && !any; ++q) {
+                    const int px = bx + (q & 7) * step, py = by + (q >> 3) * step;
+                    if (px < W && py < H) any = reaches(a1, a2, a4, o, (float)px, (float)py);
+                }
+                ++pairs;
+                kept += keep;
+                reach += any;
+                if (any && !keep) {
+                    if (bad < 5)
+                        std::printf("violation: tile %u prim %u a4=(%a %a %a %a) o=%a\n", tg, p, a4[0], a4[1],
+                                    a4[2], a4[3], o);
+                    ++bad;
+                }
+            }
+        }
+    }
+    std::printf("pairs %lld kept %lld reaching %lld violations %lld\n", (long long)pairs, (long long)kept,
+                (long long)reach, (long long)bad);
+    return bad != 0;
+}

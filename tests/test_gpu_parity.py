@@ -138,6 +138,24 @@ def test_render_is_deterministic():
     np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
+def test_blend_xcd_run_does_not_change_image():
+    """The tile-to-workgroup map is a bijection: every run length (including
+    ones longer than the tile count) gives the same image."""
+    from foveatedhtgscudabackend_amd import _lib
+    case = Case("xcd_run", 30000, 640, 480, seed=34, blur=True)
+    sc, cam, gaze, visible, tables, bg = case.build()
+    h = _lib.handle_for(0)
+    try:
+        h.set_option(_lib.OPT_BLEND_XCD_RUN, 0)
+        ref = _gpu_render(case, sc, cam, gaze, visible, bg)
+        for run in (1, 3, 16, 64, 4096):
+            h.set_option(_lib.OPT_BLEND_XCD_RUN, run)
+            got = _gpu_render(case, sc, cam, gaze, visible, bg)
+            np.testing.assert_array_equal(got.view(np.uint32), ref.view(np.uint32), err_msg=f"run={run}")
+    finally:
+        h.set_option(_lib.OPT_BLEND_XCD_RUN, 16)
+
+
 def test_stereo_shared_preprocess_equals_two_mono(oracle_lib):
     from foveatedhtgscudabackend_amd import FoveatedHTGSRasterizer
     from foveatedhtgscudabackend_amd import _lib
