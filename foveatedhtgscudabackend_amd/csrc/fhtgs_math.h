@@ -99,8 +99,9 @@ FH_HD float fh_expf(float x) {
 // operations, and the final 2^k scaling (exact in both forms) done in one
 // multiply, so the result is bit-identical to fh_expf there. The blend's
 // exponent -rho^2/2 is >= -5.55 (rho^2 <= 11.0825) or NaN.
-FH_HD float fh_expf_nonpos(float x) {
-    if (x < -64.0f) return fh_expf(x);
+// fh_expf on [-64, 0] and NaN without the range branch (the blend's
+// exponent is -rho^2/2 with rho^2 <= 11.08, so it never leaves [-6, 0]).
+FH_HD float fh_expf_m64(float x) {
     const float k = rintf(x * 0x1.715476p+0f);
     float r = fh_fma(-k, 0x1.62e400p-1f, x);
     r = fh_fma(-k, 0x1.7f7d1cp-20f, r);
@@ -114,6 +115,11 @@ FH_HD float fh_expf_nonpos(float x) {
     p = fh_fma(p, r, 1.0f);
     const int32_t ki = (k == k) ? (int32_t)k : 0;
     return p * fh_pow2i(ki);
+}
+
+FH_HD float fh_expf_nonpos(float x) {
+    if (x < -64.0f) return fh_expf(x);
+    return fh_expf_m64(x);
 }
 
 FH_HD float fh_logf(float x) {

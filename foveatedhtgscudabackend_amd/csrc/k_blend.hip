@@ -209,8 +209,9 @@ __global__ __launch_bounds__(64) void blend_kernel(ViewParams vp, BlendArgs a) {
             const float num = dot3(m, m);
             const float den = dot3(d, d);
             if (num > kMaxCutoffSq * den) continue;
-            const float rcp = 1.0f / den;
-            const float G = fh_expf_nonpos((-0.5f * num) * rcp);
+            // num <= 11.08 den here, so the exponent lies in [-6, 0] (or is NaN)
+            const float rcp = rcp_rn(den);
+            const float G = fh_expf_m64((-0.5f * num) * rcp);
             const float4 Cj = sC[j];
             const float alpha = fminf(Cj.w * G, kMaxFragmentAlpha);
             if (alpha < kMinAlpha) continue;
