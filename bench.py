@@ -122,22 +122,36 @@ def main() -> int:
         b = rast.render(*scene_t, sr, False, False, False)
         return a, b
 
-    def run(fn, steps, warmup, collect_stages):
+    def run(fn, steps, warmup, mode):
+        handle.set_option(_lib.OPT_FRAME_MODE, mode)
         for _ in range(warmup):
             fn()
         torch.cuda.synchronize()
-        lat, stages = [], []
+        lat = []
         t_start = time.perf_counter()
         for _ in range(steps):
             s0 = time.perf_counter()
             fn()
             torch.cuda.synchronize()
             lat.append((time.perf_counter() - s0) * 1e3)
-            if collect_stages:
-                stages.append(handle.stage_times())
         torch.cuda.synchronize()
         total = time.perf_counter() - t_start
-        return total, lat, stages
+        return total, lat
+
+    def stage_profile(fn, frames):
+        """Per-stage HIP-event times of the batched frame's kernels: the
+        device-count mode launches exactly what the graph replays, one event
+        pair around each stage."""
+        handle.set_option(_lib.OPT_FRAME_MODE, _lib.FRAME_DEVICE)
+        fn()
+        torch.cuda.synchronize()
+        stages = []
+        for _ in range(frames):
+            fn()
+            torch.cuda.synchronize()
+            stages.append(handle.stage_times())
+        handle.set_option(_lib.OPT_FRAME_MODE, _lib.FRAME_GRAPH)
+        return stages
 
     if args.copy_probe:
         src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
@@ -147,15 +161,21 @@ def main() -> int:
         torch.cuda.synchronize()
         del src, dst
 
-    total, lat, stages = run(stereo, args.steps, args.warmup, True)
+    # batched: both eyes in one call with one shared preprocess, no
+    # mid-frame wait, replayed from a captured hipGraph
+    total, lat = run(stereo, args.steps, args.warmup, _lib.FRAME_GRAPH)
     counts_eye = [handle.counts(0), handle.counts(1)]
     fps = args.steps / total
+    stages = stage_profile(stereo, max(5, min(args.steps, 20)))
     plain_fps = None
     plain_p50 = plain_p99 = None
     if not args.no_plain:
-        ptotal, plat, _ = run(plain, args.steps, args.warmup, False)
+        # plain: one view per call, each waiting mid-frame for its counts
+        # (the reference's per-view path)
+        ptotal, plat = run(plain, args.steps, args.warmup, _lib.FRAME_EXACT)
         plain_fps = args.steps / ptotal
         plain_p50, plain_p99 = percentile(plat, 50), percentile(plat, 99)
+        handle.set_option(_lib.OPT_FRAME_MODE, _lib.FRAME_GRAPH)
 
     stage_ms = {k: float(np.mean([s[k] for s in stages])) for k in stages[0]}
     pixels = args.width * args.height

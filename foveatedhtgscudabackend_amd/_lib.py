@@ -34,6 +34,8 @@ OPT_KEEP_UNSORTED = 1
 OPT_SORT_DIGIT_BITS = 2
 OPT_STAGE_TIMING = 3
 OPT_BLEND_XCD_RUN = 4
+OPT_FRAME_MODE = 5
+FRAME_EXACT, FRAME_DEVICE, FRAME_GRAPH = 0, 1, 2
 NUM_STAGES = 11
 STAGE_NAMES = ("clear", "tiles", "preprocess", "scan", "sync", "instances", "sort", "ranges",
                "blend", "upsample", "total")
@@ -42,6 +44,7 @@ EXPORTED_SYMBOLS = (
     "fhtgs_create", "fhtgs_destroy", "fhtgs_last_error", "fhtgs_version", "fhtgs_render",
     "fhtgs_workspace_bytes", "fhtgs_sort_pairs", "fhtgs_fovea_radius_tiles",
     "fhtgs_debug_read", "fhtgs_debug_counts", "fhtgs_set_option", "fhtgs_stage_times",
+    "fhtgs_render_batch",
 )
 
 
@@ -103,6 +106,9 @@ def load() -> ctypes.CDLL:
         lib.fhtgs_version.restype = ctypes.c_char_p
         lib.fhtgs_render.argtypes = [vp, ctypes.POINTER(Scene), ctypes.POINTER(View), i32, i32, vp]
         lib.fhtgs_render.restype = ctypes.c_int
+        lib.fhtgs_render_batch.argtypes = [vp, ctypes.POINTER(Scene), i32, ctypes.POINTER(View),
+                                           ctypes.POINTER(i32), i32, vp]
+        lib.fhtgs_render_batch.restype = ctypes.c_int
         lib.fhtgs_workspace_bytes.argtypes = [vp, ctypes.POINTER(i64)]
         lib.fhtgs_workspace_bytes.restype = ctypes.c_int
         lib.fhtgs_sort_pairs.argtypes = [vp, vp, vp, i64, i32, i32, vp]
@@ -162,6 +168,15 @@ class Handle:
             rc = self.lib.fhtgs_render(self.ptr, ctypes.byref(scene), arr, len(views),
                                        1 if share_preprocess else 0, ctypes.c_void_p(stream))
             self.check(rc, "fhtgs_render")
+
+    def render_batch(self, scenes: list, views: list, view_scene: list, stream: int) -> None:
+        sarr = (Scene * len(scenes))(*scenes)
+        varr = (View * len(views))(*views)
+        iarr = (ctypes.c_int32 * len(views))(*[int(k) for k in view_scene])
+        with self.lock:
+            rc = self.lib.fhtgs_render_batch(self.ptr, sarr, len(scenes), varr, iarr, len(views),
+                                             ctypes.c_void_p(stream))
+            self.check(rc, "fhtgs_render_batch")
 
     def sort_pairs(self, keys_ptr: int, vals_ptr: int, n: int, key_bytes: int, end_bit: int,
                    stream: int) -> None:

@@ -1,0 +1,62 @@
+"""Batched rendering: many views of several resident scenes in one call.
+
+The reference renders one view (``_C.render``) or one stereo pair
+(``_C_stereo.render``) per call. ``render_batch`` hands a whole batch to
+``fhtgs_render_batch``: consecutive views of one scene that agree on the
+preprocess parameters share one read of the Gaussians (stereo pairs), the
+frame is enqueued without a mid-frame host wait, and repeated batches of the
+same shape replay one captured hipGraph. Every image equals what
+``FoveatedHTGSRasterizer.render`` returns for that view alone.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import torch
+
+from . import _binding, _lib
+from .torch_bindings.rasterization import RasterizerSettings
+
+
+@dataclass
+class BatchView:
+    """One view of a batch: which scene, its camera settings and render flags."""
+    scene: int
+    settings: RasterizerSettings
+    to_chw: bool = False
+    blur_periphery: bool = False
+    anti_aliasing: bool = False
+    foveation_radius_px: int = _lib.DEFAULT_FOVEATION_RADIUS_PX
+    blend_width_px: int = _lib.DEFAULT_BLEND_WIDTH_PX
+    clear_output: bool = True
+
+
+def render_batch(scenes: list, views: list) -> list:
+    """scenes: list of (positions, scales, rotations, opacities, sh_0, sh_rest)
+    tuples on one GPU; views: list of BatchView. Returns one image per view
+    ((H,W,3) or (3,H,W) float32), rendered on the current stream."""
+    if not scenes or not views:
+        raise ValueError("render_batch needs at least one scene and one view")
+    keep: list = []
+    structs, device = [], None
+    for sc in scenes:
+        s, dev = _binding.scene_struct(*sc, keep=keep)
+        if device is not None and dev != device:
+            raise ValueError("all scenes of a batch must be on one device")
+        device = dev
+        structs.append(s)
+    vstructs, images, index = [], [], []
+    for bv in views:
+        if not 0 <= bv.scene < len(scenes):
+            raise ValueError(f"view refers to scene {bv.scene} of {len(scenes)}")
+        v, img = _binding.view_struct(device, *bv.settings.as_tuple(), bv.to_chw, bv.blur_periphery,
+                                      bv.anti_aliasing, bv.foveation_radius_px, bv.blend_width_px,
+                                      bv.clear_output, keep=keep)
+        vstructs.append(v)
+        images.append(img)
+        index.append(bv.scene)
+    handle = _lib.handle_for(device.index if device.index is not None else torch.cuda.current_device())
+    stream = torch.cuda.current_stream(device).cuda_stream
+    handle.render_batch(structs, vstructs, index, stream)
+    del keep
+    return images

@@ -73,8 +73,9 @@ struct ViewSlot {
     DeviceBuffer fixed;
     DeviceBuffer inst;
     DeviceBuffer debug;
-    uint32_t* host_fi = nullptr;  // pinned
-    ViewParams vp;
+    uint32_t* host_fi = nullptr;  // pinned, FI_WORDS
+    ViewParams vp;                // host copy
+    ViewParams* dvp = nullptr;    // device copy, in the handle's parameter block
     ViewBuffers vb;
     InstanceBuffers ib;
     SortPlan plan;
@@ -85,7 +86,20 @@ struct ViewSlot {
     uint32_t n_instances = 0;
     uint32_t n_active = 0;
     int64_t n_prims = 0;
+    // instance capacity for frames that do not read the counts back, from
+    // the last exact frame of the same shape (N, n_small); 0 = unknown
+    int64_t cap_hint = 0;
+    int64_t cap_for_n = -1, cap_for_tiles = -1;
 };
+
+// One preprocess launch: a scene and one or two of its views.
+struct PreGroup {
+    int scene;
+    int nv;
+    int v[2];
+};
+
+enum FrameMode { kModeExact = 0, kModeDevice = 1, kModeGraph = 2 };
 
 }  // namespace
 
@@ -100,8 +114,19 @@ struct fhtgs_context {
     int digit_bits = 11;
     int xcd_run = 16;
     int timing = 0;
+    int frame_mode = kModeGraph;
+    int last_mode = kModeExact;
     hipEvent_t ev[FHTGS_NUM_STAGES + 1] = {};
     float stage_ms[FHTGS_NUM_STAGES] = {};
+    // per-frame parameters: ViewParams of every view, then SceneArgs of every
+    // scene; written to pinned staging and copied up once per frame
+    DeviceBuffer params;
+    void* params_host = nullptr;
+    size_t params_host_cap = 0;
+    // graph replay of device-count frames
+    hipStream_t cap_stream = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<int64_t> gsig;
 };
 
 namespace {
@@ -257,6 +282,327 @@ int record(fhtgs_context* h, int stage, hipStream_t s) {
     return FHTGS_OK;
 }
 
+bool same_preprocess(const fhtgs_view& a, const fhtgs_view& b) {
+    return a.active_sh_bases == b.active_sh_bases && a.near_plane == b.near_plane &&
+           a.far_plane == b.far_plane && a.scale_modifier == b.scale_modifier &&
+           a.anti_aliasing == b.anti_aliasing;
+}
+
+int64_t with_headroom(int64_t I) {
+    const int64_t c = I + I / 4 + 65536;
+    return (c + kSortTile - 1) / kSortTile * kSortTile;
+}
+
+// Everything the host decides when it enqueues a device-count frame: the
+// same signature launches the same kernels on the same buffers, so a
+// captured graph can be replayed with only the parameter block rewritten.
+std::vector<int64_t> launch_signature(const fhtgs_context* h, const std::vector<PreGroup>& groups,
+                                      const fhtgs_scene* scenes, const fhtgs_view* views, int n_views) {
+    std::vector<int64_t> g;
+    g.push_back(n_views);
+    g.push_back(h->digit_bits);
+    g.push_back(h->xcd_run);
+    g.push_back((int64_t)(uintptr_t)h->params.ptr);
+    for (const PreGroup& pg : groups) {
+        g.push_back(pg.scene);
+        g.push_back(pg.nv);
+        g.push_back(pg.v[0]);
+        g.push_back(pg.v[1]);
+        g.push_back(scenes[pg.scene].n_primitives);
+        g.push_back(scenes[pg.scene].total_sh_bases);
+        g.push_back(views[pg.v[0]].anti_aliasing ? 1 : 0);
+    }
+    for (int v = 0; v < n_views; ++v) {
+        const ViewSlot& sl = h->slots[v];
+        g.push_back(sl.vp.width);
+        g.push_back(sl.vp.height);
+        g.push_back(views[v].K);
+        g.push_back(views[v].clear_output ? 1 : 0);
+        g.push_back(sl.ib.capacity);
+        g.push_back((int64_t)(uintptr_t)sl.fixed.ptr);
+        g.push_back((int64_t)(uintptr_t)sl.inst.ptr);
+    }
+    return g;
+}
+
+// Phase 1 of a frame: clears, tile classification, preprocess, scan.
+// dscenes[k]: device address of scene k's SceneArgs in the parameter block.
+int enqueue_phase1(fhtgs_context* h, const std::vector<PreGroup>& groups, const fhtgs_scene* scenes,
+                   const std::vector<const SceneArgs*>& dscenes, int n_views, bool timed, hipStream_t s) {
+    int rc = 0;
+    if (timed && (rc = record(h, 0, s))) return rc;
+    for (int v = 0; v < n_views; ++v) HCHK(launch_clear_frame(h->slots[v].vp, h->slots[v].vb, s));
+    if (timed && (rc = record(h, 1, s))) return rc;
+    for (int v = 0; v < n_views; ++v) HCHK(launch_tiles(h->slots[v].vp, h->slots[v].dvp, h->slots[v].vb, s));
+    if (timed && (rc = record(h, 2, s))) return rc;
+    for (const PreGroup& pg : groups) {
+        const ViewParams* hv[2] = {&h->slots[pg.v[0]].vp, &h->slots[pg.v[pg.nv - 1]].vp};
+        const ViewParams* dv[2] = {h->slots[pg.v[0]].dvp, h->slots[pg.v[pg.nv - 1]].dvp};
+        const ViewBuffers vbs[2] = {h->slots[pg.v[0]].vb, h->slots[pg.v[pg.nv - 1]].vb};
+        HCHK(launch_preprocess(hv, dv, vbs, pg.nv, dscenes[pg.scene], (uint32_t)scenes[pg.scene].n_primitives, s));
+    }
+    if (timed && (rc = record(h, 3, s))) return rc;
+    for (int v = 0; v < n_views; ++v) {
+        const int64_t N = h->slots[v].n_prims;
+        if (N > 0) HCHK(launch_scan_blocks(h->slots[v].vb, (uint32_t)((N + kPrimBlock - 1) / kPrimBlock), s));
+    }
+    return FHTGS_OK;
+}
+
+// Phase 2: instances, sort, ranges, output clear + blend, upsample; each
+// stage for every view before the next, so one event pair times a stage.
+// exact: grids from the counts read back; otherwise from the capacities.
+int enqueue_phase2(fhtgs_context* h, const fhtgs_view* views, int n_views, bool exact, bool timed,
+                   hipStream_t s) {
+    int rc = 0;
+    if (timed && (rc = record(h, 5, s))) return rc;
+    for (int v = 0; v < n_views; ++v) {
+        ViewSlot& slot = h->slots[v];
+        const int64_t tiles = exact ? ((int64_t)slot.n_instances + kSortTile - 1) / kSortTile : slot.ib.tiles_cap;
+        HCHK(launch_sort_clear(slot.ib, slot.plan, tiles, s));
+        HCHK(launch_instances(slot.dvp, slot.vb, slot.ib, slot.plan, (uint32_t)slot.n_prims, s));
+        if (exact && h->keep_unsorted && slot.n_instances > 0) {
+            HCHK(launch_widen_keys(slot.ib.keys[0], slot.plan.key_bytes, slot.dbg_keys, slot.n_instances, s));
+            HCHK(hipMemcpyAsync(slot.dbg_vals, slot.ib.vals[0], (size_t)slot.n_instances * 4,
+                                hipMemcpyDeviceToDevice, s));
+        }
+    }
+    if (timed && (rc = record(h, 6, s))) return rc;
+    for (int v = 0; v < n_views; ++v) {
+        ViewSlot& slot = h->slots[v];
+        const int64_t tiles = exact ? ((int64_t)slot.n_instances + kSortTile - 1) / kSortTile : slot.ib.tiles_cap;
+        int src = 0;
+        for (int p = 0; p < slot.plan.passes; ++p) {
+            HCHK(launch_sort_pass(slot.ib, slot.plan, p, src, tiles, slot.vb.frame_info, 0u, s));
+            src = 1 - src;
+        }
+        slot.sorted_src = src;
+    }
+    if (timed && (rc = record(h, 7, s))) return rc;
+    for (int v = 0; v < n_views; ++v) {
+        ViewSlot& slot = h->slots[v];
+        HCHK(launch_ranges(slot.vb, slot.ib, slot.plan, slot.sorted_src,
+                           exact ? slot.n_instances : (uint32_t)slot.ib.capacity, s));
+    }
+    if (timed && (rc = record(h, 8, s))) return rc;
+    for (int v = 0; v < n_views; ++v) {
+        ViewSlot& slot = h->slots[v];
+        if (views[v].clear_output) HCHK(launch_clear_image(slot.vp, slot.dvp, slot.vb, s));
+        HCHK(launch_blend(slot.dvp, slot.vb, slot.ib.vals[slot.sorted_src], views[v].K,
+                          exact ? slot.n_active : (uint32_t)slot.vp.n_small, h->xcd_run, s));
+    }
+    if (timed && (rc = record(h, 9, s))) return rc;
+    for (int v = 0; v < n_views; ++v) {
+        ViewSlot& slot = h->slots[v];
+        uint32_t grid = (uint32_t)slot.vp.n_large;
+        if (exact) {
+            const uint32_t* part = slot.host_fi + FI_PART;
+            grid = (part[3] - part[2]) + (part[5] - part[4] + 3u) / 4u;
+        }
+        HCHK(launch_upsample(slot.dvp, slot.vb, grid, s));
+    }
+    if (timed && (rc = record(h, 10, s))) return rc;
+    return FHTGS_OK;
+}
+
+// Instance buffers of a view for `cap` instances.
+int size_instances(fhtgs_context* h, ViewSlot& slot, int64_t cap) {
+    const int end_bit = end_bit_of((uint32_t)slot.vp.n_small + 1u);
+    slot.plan = make_plan(end_bit, h->digit_bits);
+    size_t bytes = 0;
+    carve_inst(nullptr, cap, slot.plan, slot.ib, bytes);
+    int rc = ensure(h, slot.inst, bytes);
+    if (rc) return rc;
+    carve_inst(slot.inst.ptr, cap, slot.plan, slot.ib, bytes);
+    return FHTGS_OK;
+}
+
+int render_impl(fhtgs_context* h, const fhtgs_scene* scenes, int n_scenes, const fhtgs_view* views,
+                const int32_t* view_scene, int n_views, bool pair_views, hipStream_t s, int force_mode) {
+    // ---- preprocess groups: consecutive views of one scene that agree on
+    // the preprocess parameters share one pass
+    std::vector<PreGroup> groups;
+    for (int v = 0; v < n_views;) {
+        PreGroup pg{view_scene[v], 1, {v, v}};
+        if (pair_views && v + 1 < n_views && view_scene[v + 1] == view_scene[v] &&
+            same_preprocess(views[v], views[v + 1])) {
+            pg.nv = 2;
+            pg.v[1] = v + 1;
+        }
+        groups.push_back(pg);
+        v += pg.nv;
+    }
+    if ((int)h->slots.size() < n_views) h->slots.resize(n_views);
+
+    // ---- parameters: host copies, then one upload
+    const size_t vp_stride = align_up(sizeof(ViewParams));
+    const size_t sc_stride = align_up(sizeof(SceneArgs));
+    const size_t pbytes = vp_stride * (size_t)n_views + sc_stride * (size_t)n_scenes;
+    int rc = ensure(h, h->params, pbytes);
+    if (rc) return rc;
+    if (h->params_host_cap < pbytes) {
+        if (h->params_host) HCHK(hipHostFree(h->params_host));
+        h->params_host = nullptr;
+        h->params_host_cap = 0;
+        HCHK(hipHostMalloc(&h->params_host, h->params.cap, hipHostMallocDefault));
+        h->params_host_cap = h->params.cap;
+    }
+    char* ph = (char*)h->params_host;
+    char* pd = (char*)h->params.ptr;
+    // scene k at byte offset vp_stride * n_views + sc_stride * k, in the
+    // staging copy and on the device alike
+    std::vector<const SceneArgs*> dscenes((size_t)n_scenes);
+    for (int k = 0; k < n_scenes; ++k) {
+        const fhtgs_scene& sc = scenes[k];
+        const SceneArgs a{sc.positions, sc.scales, sc.rotations, sc.opacities, sc.sh_0, sc.sh_rest};
+        const size_t off = vp_stride * (size_t)n_views + sc_stride * (size_t)k;
+        std::memcpy(ph + off, &a, sizeof(a));
+        dscenes[(size_t)k] = (const SceneArgs*)(pd + off);
+    }
+    for (int v = 0; v < n_views; ++v) {
+        ViewSlot& slot = h->slots[v];
+        const fhtgs_scene& sc = scenes[view_scene[v]];
+        fill_params(sc, views[v], slot.vp);
+        std::memcpy(ph + vp_stride * (size_t)v, &slot.vp, sizeof(ViewParams));
+        slot.dvp = (ViewParams*)(pd + vp_stride * (size_t)v);
+        const int64_t N = sc.n_primitives;
+        size_t bytes = 0;
+        carve_fixed(nullptr, N, slot.vp.n_large, slot.vp.n_small, slot.vb, bytes);
+        if ((rc = ensure(h, slot.fixed, bytes))) return rc;
+        carve_fixed(slot.fixed.ptr, N, slot.vp.n_large, slot.vp.n_small, slot.vb, bytes);
+        if (!slot.host_fi) HCHK(hipHostMalloc((void**)&slot.host_fi, FI_WORDS * sizeof(uint32_t), hipHostMallocDefault));
+        slot.n_prims = N;
+    }
+
+    // ---- mode: device counts need a capacity from an exact frame of this shape
+    int mode = force_mode >= 0 ? force_mode : h->frame_mode;
+    if (h->keep_unsorted) mode = kModeExact;
+    for (int v = 0; v < n_views && mode != kModeExact; ++v) {
+        const ViewSlot& slot = h->slots[v];
+        if (slot.cap_hint <= 0 || slot.cap_for_n != slot.n_prims || slot.cap_for_tiles != slot.vp.n_small)
+            mode = kModeExact;
+    }
+    if (mode != kModeExact) {
+        for (int v = 0; v < n_views; ++v)
+            if ((rc = size_instances(h, h->slots[v], h->slots[v].cap_hint))) return rc;
+    }
+    const bool timed = h->timing != 0 && mode != kModeGraph;
+    HCHK(hipMemcpyAsync(h->params.ptr, h->params_host, pbytes, hipMemcpyHostToDevice, s));
+
+    if (mode == kModeExact) {
+        if ((rc = enqueue_phase1(h, groups, scenes, dscenes, n_views, timed, s))) return rc;
+        for (int v = 0; v < n_views; ++v)
+            HCHK(hipMemcpyAsync(h->slots[v].host_fi, h->slots[v].vb.frame_info, FI_WORDS * sizeof(uint32_t),
+                                hipMemcpyDeviceToHost, s));
+        if (timed && (rc = record(h, 4, s))) return rc;
+        HCHK(hipStreamSynchronize(s));
+        for (int v = 0; v < n_views; ++v) {
+            ViewSlot& slot = h->slots[v];
+            const uint32_t I = slot.n_prims > 0 ? slot.host_fi[FI_INSTANCES] : 0u;
+            if (I >= (1u << 30)) return set_err(h, FHTGS_E_INVALID, "more than 2^30 tile instances in one view");
+            slot.n_instances = I;
+            slot.n_active = slot.host_fi[FI_N_ACTIVE];
+            if ((rc = size_instances(h, slot, I))) return rc;
+            if (h->keep_unsorted) {
+                const size_t db = align_up((size_t)std::max<uint32_t>(I, 1) * 4) * 2;
+                if ((rc = ensure(h, slot.debug, db))) return rc;
+                slot.dbg_keys = (uint32_t*)slot.debug.ptr;
+                slot.dbg_vals = (uint32_t*)((char*)slot.debug.ptr + db / 2);
+            }
+        }
+        if ((rc = enqueue_phase2(h, views, n_views, true, timed, s))) return rc;
+    } else if (mode == kModeDevice) {
+        if ((rc = enqueue_phase1(h, groups, scenes, dscenes, n_views, timed, s))) return rc;
+        if (timed && (rc = record(h, 4, s))) return rc;
+        if ((rc = enqueue_phase2(h, views, n_views, false, timed, s))) return rc;
+    } else {
+        if (h->timing) HCHK(hipEventRecord(h->ev[0], s));
+        std::vector<int64_t> sig = launch_signature(h, groups, scenes, views, n_views);
+        if (!h->gexec || sig != h->gsig) {
+            if (h->gexec) {
+                HCHK(hipGraphExecDestroy(h->gexec));
+                h->gexec = nullptr;
+            }
+            h->gsig.clear();
+            HCHK(hipStreamBeginCapture(h->cap_stream, hipStreamCaptureModeThreadLocal));
+            int crc = enqueue_phase1(h, groups, scenes, dscenes, n_views, false, h->cap_stream);
+            if (!crc) crc = enqueue_phase2(h, views, n_views, false, false, h->cap_stream);
+            hipGraph_t graph = nullptr;
+            const hipError_t ce = hipStreamEndCapture(h->cap_stream, &graph);
+            if (crc) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return crc;
+            }
+            if (ce != hipSuccess) return set_err(h, FHTGS_E_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
+            const hipError_t ie = hipGraphInstantiate(&h->gexec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ie != hipSuccess) {
+                h->gexec = nullptr;
+                return set_err(h, FHTGS_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
+            }
+            h->gsig = sig;
+        }
+        HCHK(hipGraphLaunch(h->gexec, s));
+        for (int v = 0; v < n_views; ++v) h->slots[v].sorted_src = h->slots[v].plan.passes & 1;
+        if (h->timing) HCHK(hipEventRecord(h->ev[FHTGS_NUM_STAGES - 1], s));
+    }
+    // frame info at the end: counts, valid and visible totals, error bits
+    for (int v = 0; v < n_views; ++v)
+        HCHK(hipMemcpyAsync(h->slots[v].host_fi, h->slots[v].vb.frame_info, FI_WORDS * sizeof(uint32_t),
+                            hipMemcpyDeviceToHost, s));
+    HCHK(hipEventRecord(h->done, s));
+    h->have_done = true;
+    // the reference call returns after its streams finish (inference.cu:468-470)
+    HCHK(hipStreamSynchronize(s));
+    h->last_mode = mode;
+
+    bool redo = false;
+    for (int v = 0; v < n_views; ++v) {
+        ViewSlot& slot = h->slots[v];
+        const uint32_t* fi = slot.host_fi;
+        if (fi[FI_ERROR] & 1u)
+            return set_err(h, FHTGS_E_TABLES,
+                           "tiles-touched from the culling tables under-counts the visible tiles of a Gaussian");
+        if (fi[FI_ERROR] & 4u) {
+            redo = true;
+            continue;
+        }
+        if (fi[FI_ERROR] & (uint32_t)kSortErrorBit)
+            return set_err(h, FHTGS_E_HIP, "instance sort detected an inconsistency (look-back or scatter)");
+        const uint32_t I = slot.n_prims > 0 ? fi[FI_INSTANCES] : 0u;
+        slot.n_instances = I;
+        slot.n_active = fi[FI_N_ACTIVE];
+        if (mode == kModeExact || (int64_t)I > slot.ib.capacity - slot.ib.capacity / 8) {
+            slot.cap_hint = with_headroom(I);
+            slot.cap_for_n = slot.n_prims;
+            slot.cap_for_tiles = slot.vp.n_small;
+        }
+        int64_t* c = slot.counts;
+        c[0] = slot.n_prims; c[1] = slot.vp.n_large; c[2] = slot.vp.n_small; c[3] = slot.n_active;
+        c[4] = I; c[5] = fi[FI_VALID]; c[6] = slot.plan.end_bit; c[7] = slot.plan.key_bytes;
+        c[8] = fi[FI_N_VISIBLE];
+        c[9] = slot.vp.gw; c[10] = slot.vp.gh; c[11] = slot.vp.fovea_r; c[12] = slot.vp.blend_r;
+        c[13] = slot.vp.gtx; c[14] = slot.vp.gty; c[15] = slot.plan.passes;
+    }
+    if (redo) {
+        // a view outgrew its instance capacity: render the frame again with
+        // exact buffers (the next frames use the new capacity)
+        return render_impl(h, scenes, n_scenes, views, view_scene, n_views, pair_views, s, kModeExact);
+    }
+    if (h->timing) {
+        // graph frames time the whole frame only
+        for (int i = 0; i < FHTGS_NUM_STAGES - 1; ++i) {
+            float ms = 0.0f;
+            if (mode != kModeGraph) HCHK(hipEventElapsedTime(&ms, h->ev[i], h->ev[i + 1]));
+            h->stage_ms[i] = ms;
+        }
+        float tot = 0.0f;
+        HCHK(hipEventElapsedTime(&tot, h->ev[0], h->ev[FHTGS_NUM_STAGES - 1]));
+        h->stage_ms[FHTGS_NUM_STAGES - 1] = tot;
+    }
+    return FHTGS_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -285,6 +631,10 @@ int fhtgs_create(int device, fhtgs_handle* out) {
             return FHTGS_E_HIP;
         }
     }
+    if (hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking) != hipSuccess) {
+        delete h;
+        return FHTGS_E_HIP;
+    }
     (void)hipSetDevice(prev);
     *out = h;
     return FHTGS_OK;
@@ -303,6 +653,10 @@ int fhtgs_destroy(fhtgs_handle h) {
         if (s.host_fi) (void)hipHostFree(s.host_fi);
     }
     if (h->sort_scratch.ptr) (void)hipFree(h->sort_scratch.ptr);
+    if (h->params.ptr) (void)hipFree(h->params.ptr);
+    if (h->params_host) (void)hipHostFree(h->params_host);
+    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+    if (h->cap_stream) (void)hipStreamDestroy(h->cap_stream);
     if (h->done) (void)hipEventDestroy(h->done);
     for (int i = 0; i <= FHTGS_NUM_STAGES; ++i)
         if (h->ev[i]) (void)hipEventDestroy(h->ev[i]);
@@ -332,6 +686,10 @@ int fhtgs_set_option(fhtgs_handle h, int32_t option, int64_t value) {
             h->digit_bits = (int)value;
             return FHTGS_OK;
         case FHTGS_OPT_STAGE_TIMING: h->timing = value ? 1 : 0; return FHTGS_OK;
+        case FHTGS_OPT_FRAME_MODE:
+            if (value < 0 || value > 2) return set_err(h, FHTGS_E_INVALID, "frame mode must be 0, 1 or 2");
+            h->frame_mode = (int)value;
+            return FHTGS_OK;
         case FHTGS_OPT_BLEND_XCD_RUN:
             if (value < 0 || value > 4096) return set_err(h, FHTGS_E_INVALID, "XCD run must be 0..4096");
             h->xcd_run = (int)value;
@@ -342,7 +700,7 @@ int fhtgs_set_option(fhtgs_handle h, int32_t option, int64_t value) {
 
 int fhtgs_workspace_bytes(fhtgs_handle h, int64_t* bytes) {
     if (!h || !bytes) return FHTGS_E_INVALID;
-    int64_t t = (int64_t)h->sort_scratch.cap;
+    int64_t t = (int64_t)(h->sort_scratch.cap + h->params.cap);
     for (auto& s : h->slots) t += (int64_t)(s.fixed.cap + s.inst.cap + s.debug.cap);
     *bytes = t;
     return FHTGS_OK;
@@ -354,34 +712,32 @@ int fhtgs_stage_times(fhtgs_handle h, float* ms) {
     return FHTGS_OK;
 }
 
-int fhtgs_render(fhtgs_handle h, const fhtgs_scene* scene, const fhtgs_view* views, int32_t n_views,
-                 int32_t share_preprocess, void* stream_ptr) {
-    if (!h) return FHTGS_E_INVALID;
-    if (!scene || !views || n_views <= 0) return set_err(h, FHTGS_E_INVALID, "null scene/views or no view");
-    if (share_preprocess && n_views > 2)
-        return set_err(h, FHTGS_E_INVALID, "a shared preprocess covers at most two views");
-    const fhtgs_scene& sc = *scene;
+namespace {
+
+int check_scene(fhtgs_context* h, const fhtgs_scene& sc, int idx) {
+    const std::string p = "scene " + std::to_string(idx) + ": ";
     if (sc.n_primitives < 0 || sc.n_primitives >= ((int64_t)1 << 31))
-        return set_err(h, FHTGS_E_INVALID, "primitive count out of range");
+        return set_err(h, FHTGS_E_INVALID, p + "primitive count out of range");
     if (sc.n_primitives > 0 && (!sc.positions || !sc.scales || !sc.rotations || !sc.opacities || !sc.sh_0 ||
                                 (!sc.sh_rest && sc.total_sh_bases > 0)))
-        return set_err(h, FHTGS_E_INVALID, "null scene tensor");
+        return set_err(h, FHTGS_E_INVALID, p + "null scene tensor");
+    return FHTGS_OK;
+}
+
+int render_entry(fhtgs_context* h, const fhtgs_scene* scenes, int n_scenes, const fhtgs_view* views,
+                 const int32_t* view_scene, int n_views, bool pair_views, hipStream_t s) {
+    for (int k = 0; k < n_scenes; ++k) {
+        const int rc = check_scene(h, scenes[k], k);
+        if (rc) return rc;
+    }
     for (int v = 0; v < n_views; ++v) {
         const int rc = validate_view(h, views[v], v);
         if (rc) return rc;
-        if (views[v].active_sh_bases > 1 && views[v].active_sh_bases > sc.total_sh_bases + 1)
+        if (view_scene[v] < 0 || view_scene[v] >= n_scenes)
+            return set_err(h, FHTGS_E_INVALID, "view " + std::to_string(v) + ": scene index out of range");
+        if (views[v].active_sh_bases > 1 && views[v].active_sh_bases > scenes[view_scene[v]].total_sh_bases + 1)
             return set_err(h, FHTGS_E_INVALID, "active_sh_bases exceeds the SH coefficients given");
     }
-    if (share_preprocess) {
-        const fhtgs_view& a = views[0];
-        for (int v = 1; v < n_views; ++v) {
-            const fhtgs_view& b = views[v];
-            if (a.active_sh_bases != b.active_sh_bases || a.near_plane != b.near_plane ||
-                a.far_plane != b.far_plane || a.scale_modifier != b.scale_modifier || a.anti_aliasing != b.anti_aliasing)
-                return set_err(h, FHTGS_E_INVALID, "shared-preprocess views disagree on preprocess parameters");
-        }
-    }
-    hipStream_t s = (hipStream_t)stream_ptr;
     int prev_dev = 0;
     HCHK(hipGetDevice(&prev_dev));
     if (prev_dev != h->device) HCHK(hipSetDevice(h->device));
@@ -389,147 +745,31 @@ int fhtgs_render(fhtgs_handle h, const fhtgs_scene* scene, const fhtgs_view* vie
         int d, cur;
         ~Restore() { if (d != cur) (void)hipSetDevice(d); }
     } restore{prev_dev, h->device};
-
     if (h->have_done) HCHK(hipStreamWaitEvent(s, h->done, 0));
-    if ((int)h->slots.size() < n_views) h->slots.resize(n_views);
-    const int64_t N = sc.n_primitives;
-    const uint32_t nb = (uint32_t)((N + kPrimBlock - 1) / kPrimBlock);
+    return render_impl(h, scenes, n_scenes, views, view_scene, n_views, pair_views, s, -1);
+}
 
-    // ---- phase 1: tiles, preprocess, scan
-    for (int v = 0; v < n_views; ++v) {
-        ViewSlot& slot = h->slots[v];
-        fill_params(sc, views[v], slot.vp);
-        size_t bytes = 0;
-        carve_fixed(nullptr, N, slot.vp.n_large, slot.vp.n_small, slot.vb, bytes);
-        int rc = ensure(h, slot.fixed, bytes);
-        if (rc) return rc;
-        carve_fixed(slot.fixed.ptr, N, slot.vp.n_large, slot.vp.n_small, slot.vb, bytes);
-        if (!slot.host_fi) HCHK(hipHostMalloc((void**)&slot.host_fi, FI_WORDS * sizeof(uint32_t), hipHostMallocDefault));
-        slot.n_prims = N;
-    }
-    int rc = record(h, 0, s);
-    if (rc) return rc;
-    for (int v = 0; v < n_views; ++v) HCHK(launch_clear_frame(h->slots[v].vp, h->slots[v].vb, s));
-    if ((rc = record(h, 1, s))) return rc;
-    for (int v = 0; v < n_views; ++v) HCHK(launch_tiles(h->slots[v].vp, h->slots[v].vb, s));
-    if ((rc = record(h, 2, s))) return rc;
-    if (share_preprocess && n_views == 2) {
-        ViewParams vps[2] = {h->slots[0].vp, h->slots[1].vp};
-        ViewBuffers vbs[2] = {h->slots[0].vb, h->slots[1].vb};
-        HCHK(launch_preprocess(vps, vbs, 2, sc.positions, sc.scales, sc.rotations, sc.opacities, sc.sh_0,
-                               sc.sh_rest, (uint32_t)N, 0, s));
-    } else {
-        for (int v = 0; v < n_views; ++v)
-            HCHK(launch_preprocess(&h->slots[v].vp, &h->slots[v].vb, 1, sc.positions, sc.scales, sc.rotations,
-                                   sc.opacities, sc.sh_0, sc.sh_rest, (uint32_t)N, 0, s));
-    }
-    if ((rc = record(h, 3, s))) return rc;
-    for (int v = 0; v < n_views; ++v) {
-        ViewSlot& slot = h->slots[v];
-        if (N > 0) HCHK(launch_scan_blocks(slot.vb, nb, s));
-        HCHK(hipMemcpyAsync(slot.host_fi, slot.vb.frame_info, FI_WORDS * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    }
-    if ((rc = record(h, 4, s))) return rc;
-    HCHK(hipStreamSynchronize(s));
+}  // namespace
 
-    // ---- phase 2: instances, sort, ranges, blend, upsample; each stage for
-    // every view before the next, so one event pair times a stage
-    if ((rc = record(h, 5, s))) return rc;
-    std::vector<int64_t> tiles_of(n_views, 0);
-    for (int v = 0; v < n_views; ++v) {
-        ViewSlot& slot = h->slots[v];
-        const uint32_t* fi = slot.host_fi;
-        const uint32_t I = N > 0 ? fi[FI_INSTANCES] : 0u;
-        if (I >= (1u << 30)) return set_err(h, FHTGS_E_INVALID, "more than 2^30 tile instances in one view");
-        slot.n_instances = I;
-        slot.n_active = fi[FI_N_ACTIVE];
-        const int end_bit = end_bit_of((uint32_t)slot.vp.n_small + 1u);
-        slot.plan = make_plan(end_bit, h->digit_bits);
-        size_t bytes = 0;
-        carve_inst(nullptr, I, slot.plan, slot.ib, bytes);
-        if ((rc = ensure(h, slot.inst, bytes))) return rc;
-        carve_inst(slot.inst.ptr, I, slot.plan, slot.ib, bytes);
-        tiles_of[v] = ((int64_t)I + kSortTile - 1) / kSortTile;
-        if (h->keep_unsorted) {
-            const size_t db = align_up((size_t)std::max<uint32_t>(I, 1) * 4) * 2;
-            if ((rc = ensure(h, slot.debug, db))) return rc;
-            slot.dbg_keys = (uint32_t*)slot.debug.ptr;
-            slot.dbg_vals = (uint32_t*)((char*)slot.debug.ptr + db / 2);
-        }
-        int64_t* c = slot.counts;
-        c[0] = N; c[1] = slot.vp.n_large; c[2] = slot.vp.n_small; c[3] = slot.n_active;
-        c[4] = I; c[5] = -1; c[6] = end_bit; c[7] = slot.plan.key_bytes; c[8] = -1;
-        c[9] = slot.vp.gw; c[10] = slot.vp.gh; c[11] = slot.vp.fovea_r; c[12] = slot.vp.blend_r;
-        c[13] = slot.vp.gtx; c[14] = slot.vp.gty; c[15] = slot.plan.passes;
-    }
-    for (int v = 0; v < n_views; ++v) {
-        ViewSlot& slot = h->slots[v];
-        HCHK(launch_sort_clear(slot.ib, slot.plan, tiles_of[v], s));
-        HCHK(launch_instances(slot.vp, slot.vb, slot.ib, slot.plan, (uint32_t)N, s));
-        if (h->keep_unsorted && slot.n_instances > 0) {
-            HCHK(launch_widen_keys(slot.ib.keys[0], slot.plan.key_bytes, slot.dbg_keys, slot.n_instances, s));
-            HCHK(hipMemcpyAsync(slot.dbg_vals, slot.ib.vals[0], (size_t)slot.n_instances * 4,
-                                hipMemcpyDeviceToDevice, s));
-        }
-    }
-    if ((rc = record(h, 6, s))) return rc;
-    for (int v = 0; v < n_views; ++v) {
-        ViewSlot& slot = h->slots[v];
-        int src = 0;
-        for (int p = 0; p < slot.plan.passes; ++p) {
-            HCHK(launch_sort_pass(slot.ib, slot.plan, p, src, tiles_of[v], slot.vb.frame_info, 0u, s));
-            src = 1 - src;
-        }
-        slot.sorted_src = src;
-    }
-    if ((rc = record(h, 7, s))) return rc;
-    for (int v = 0; v < n_views; ++v) {
-        ViewSlot& slot = h->slots[v];
-        HCHK(launch_ranges(slot.vb, slot.ib, slot.plan, slot.sorted_src, slot.n_instances, s));
-    }
-    if ((rc = record(h, 8, s))) return rc;
-    for (int v = 0; v < n_views; ++v) {
-        ViewSlot& slot = h->slots[v];
-        if (views[v].clear_output)
-            HCHK(launch_clear_image(slot.vp, slot.vb, slot.n_active <= 1 ? 1 : 0, s));
-        HCHK(launch_blend(slot.vp, slot.vb, slot.ib.vals[slot.sorted_src], views[v].K, slot.n_active, h->xcd_run, s));
-    }
-    if ((rc = record(h, 9, s))) return rc;
-    for (int v = 0; v < n_views; ++v) {
-        ViewSlot& slot = h->slots[v];
-        const uint32_t* part = slot.host_fi + FI_PART;
-        HCHK(launch_upsample(slot.vp, slot.vb, part[3] - part[2], part[5] - part[4], s));
-    }
-    if ((rc = record(h, 10, s))) return rc;
-    HCHK(hipEventRecord(h->done, s));
-    h->have_done = true;
-    // the reference call returns after its streams finish (inference.cu:468-470)
-    HCHK(hipStreamSynchronize(s));
-    for (int v = 0; v < n_views; ++v) {
-        ViewSlot& slot = h->slots[v];
-        HCHK(hipMemcpy(slot.host_fi, slot.vb.frame_info, FI_WORDS * sizeof(uint32_t), hipMemcpyDeviceToHost));
-        slot.counts[5] = slot.host_fi[FI_VALID];
-        slot.counts[8] = slot.host_fi[FI_N_VISIBLE];
-        if (slot.host_fi[FI_ERROR] & 1u)
-            return set_err(h, FHTGS_E_TABLES,
-                           "tiles-touched from the culling tables under-counts the visible tiles of a Gaussian");
-        uint32_t sort_err = 0;
-        if (slot.n_instances > 0)
-            HCHK(hipMemcpy(&sort_err, slot.ib.tickets + kMaxPasses, sizeof(uint32_t), hipMemcpyDeviceToHost));
-        if (sort_err)
-            return set_err(h, FHTGS_E_HIP, "instance sort detected an inconsistency (look-back or scatter)");
-    }
-    if (h->timing) {
-        for (int i = 0; i < FHTGS_NUM_STAGES - 1; ++i) {
-            float ms = 0.0f;
-            HCHK(hipEventElapsedTime(&ms, h->ev[i], h->ev[i + 1]));
-            h->stage_ms[i] = ms;
-        }
-        float tot = 0.0f;
-        HCHK(hipEventElapsedTime(&tot, h->ev[0], h->ev[FHTGS_NUM_STAGES - 1]));
-        h->stage_ms[FHTGS_NUM_STAGES - 1] = tot;
-    }
-    return FHTGS_OK;
+int fhtgs_render(fhtgs_handle h, const fhtgs_scene* scene, const fhtgs_view* views, int32_t n_views,
+                 int32_t share_preprocess, void* stream_ptr) {
+    if (!h) return FHTGS_E_INVALID;
+    if (!scene || !views || n_views <= 0) return set_err(h, FHTGS_E_INVALID, "null scene/views or no view");
+    if (share_preprocess && n_views > 2)
+        return set_err(h, FHTGS_E_INVALID, "a shared preprocess covers at most two views");
+    if (share_preprocess && n_views == 2 && !same_preprocess(views[0], views[1]))
+        return set_err(h, FHTGS_E_INVALID, "shared-preprocess views disagree on preprocess parameters");
+    std::vector<int32_t> vs((size_t)n_views, 0);
+    return render_entry(h, scene, 1, views, vs.data(), n_views, share_preprocess != 0,
+                        (hipStream_t)stream_ptr);
+}
+
+int fhtgs_render_batch(fhtgs_handle h, const fhtgs_scene* scenes, int32_t n_scenes, const fhtgs_view* views,
+                       const int32_t* view_scene, int32_t n_views, void* stream_ptr) {
+    if (!h) return FHTGS_E_INVALID;
+    if (!scenes || n_scenes <= 0 || !views || !view_scene || n_views <= 0)
+        return set_err(h, FHTGS_E_INVALID, "null scenes/views/view_scene or empty batch");
+    return render_entry(h, scenes, n_scenes, views, view_scene, n_views, true, (hipStream_t)stream_ptr);
 }
 
 int fhtgs_debug_counts(fhtgs_handle h, int32_t view, int64_t* counts) {

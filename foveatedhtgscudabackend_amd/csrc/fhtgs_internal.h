@@ -45,7 +45,7 @@ enum FrameInfo : int {
     FI_N_ACTIVE = 6,
     FI_INSTANCES = 7,     // I = sum of tiles-touched
     FI_VALID = 8,         // non-sentinel instances (atomic)
-    FI_ERROR = 9,         // bit 0: tiles-touched under-count, bit 1: sort inconsistency
+    FI_ERROR = 9,         // bit 0: tiles-touched under-count, bit 2: instance capacity exceeded
     FI_N_VISIBLE = 10,    // preprocess survivors (atomic)
     FI_WORDS = 16
 };
@@ -69,6 +69,17 @@ struct ViewParams {
     int to_chw, blur, aaa;
     float* image_final;   // fovea tiles and all upsampled output
     float* image_blend;   // periphery / blended blend target (temp image when blurring)
+};
+
+// Scene tensors of one preprocess launch (device resident, like ViewParams,
+// so that a captured frame can be replayed with other inputs).
+struct SceneArgs {
+    const float* pos;
+    const float* scl;
+    const float* rot;
+    const float* opa;
+    const float* sh0;
+    const float* shr;
 };
 
 // Per-view device buffers for one frame.
@@ -100,21 +111,25 @@ struct InstanceBuffers {
     uint32_t* hist;      // kMaxPasses * kMaxBins
     uint32_t* status;    // passes * tiles * bins_max
     uint32_t* tickets;   // kMaxPasses tickets, then one error word
-    int64_t capacity;    // instances
+    int64_t capacity;    // instances; the instance, sort and range kernels stop
+                         // (FI_ERROR bit 2) when a frame needs more
     int64_t tiles_cap;   // onesweep tiles
 };
 
 // ---- launchers (one translation unit per kernel family) ----
+// Kernels read their view parameters through `dvp`, a device copy of the
+// host ViewParams, and the frame's counts (instances, active tiles, tile-type
+// partition) from the frame-info words; the host only sizes the grids, either
+// exactly (after reading the counts back) or to the buffers' capacity.
 hipError_t launch_clear_frame(const ViewParams& vp, const ViewBuffers& vb, hipStream_t s);
-hipError_t launch_tiles(const ViewParams& vp, const ViewBuffers& vb, hipStream_t s);
-hipError_t launch_preprocess(const ViewParams* vps, const ViewBuffers* vbs, int n_views,
-                             const float* pos, const float* scl, const float* rot,
-                             const float* opa, const float* sh0, const float* shr,
-                             uint32_t n, int K_unused, hipStream_t s);
+hipError_t launch_tiles(const ViewParams& vp, const ViewParams* dvp, const ViewBuffers& vb, hipStream_t s);
+hipError_t launch_preprocess(const ViewParams* const* vps, const ViewParams* const* dvps,
+                             const ViewBuffers* vbs, int n_views, const SceneArgs* dscene,
+                             uint32_t n, hipStream_t s);
 hipError_t launch_scan_blocks(const ViewBuffers& vb, uint32_t n_blocks, hipStream_t s);
 hipError_t launch_sort_clear(const InstanceBuffers& ib, const SortPlan& plan, int64_t tiles,
                              hipStream_t s);
-hipError_t launch_instances(const ViewParams& vp, const ViewBuffers& vb, const InstanceBuffers& ib,
+hipError_t launch_instances(const ViewParams* dvp, const ViewBuffers& vb, const InstanceBuffers& ib,
                             const SortPlan& plan, uint32_t n, hipStream_t s);
 hipError_t launch_sort_pass(const InstanceBuffers& ib, const SortPlan& plan, int pass, int src,
                             int64_t n_tiles, const uint32_t* frame_info, uint32_t n_explicit,
@@ -122,12 +137,11 @@ hipError_t launch_sort_pass(const InstanceBuffers& ib, const SortPlan& plan, int
 hipError_t launch_sort_histogram(const InstanceBuffers& ib, const SortPlan& plan, int src,
                                  uint32_t n, hipStream_t s);
 hipError_t launch_ranges(const ViewBuffers& vb, const InstanceBuffers& ib, const SortPlan& plan,
-                         int src, uint32_t n, hipStream_t s);
-hipError_t launch_blend(const ViewParams& vp, const ViewBuffers& vb, const uint32_t* sorted_vals,
-                        int K, uint32_t n_active, int xcd_run, hipStream_t s);
-hipError_t launch_upsample(const ViewParams& vp, const ViewBuffers& vb, uint32_t n_periphery,
-                           uint32_t n_blended, hipStream_t s);
-hipError_t launch_clear_image(const ViewParams& vp, const ViewBuffers& vb, int clear_all,
+                         int src, uint32_t n_grid, hipStream_t s);
+hipError_t launch_blend(const ViewParams* dvp, const ViewBuffers& vb, const uint32_t* sorted_vals,
+                        int K, uint32_t grid, int xcd_run, hipStream_t s);
+hipError_t launch_upsample(const ViewParams* dvp, const ViewBuffers& vb, uint32_t grid, hipStream_t s);
+hipError_t launch_clear_image(const ViewParams& vp, const ViewParams* dvp, const ViewBuffers& vb,
                               hipStream_t s);
 hipError_t launch_widen_keys(const void* keys, int key_bytes, uint32_t* out, uint32_t n,
                              hipStream_t s);

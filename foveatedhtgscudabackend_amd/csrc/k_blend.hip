@@ -134,7 +134,8 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n, uint32_t r
 }
 
 template <int K>
-__global__ __launch_bounds__(64) void blend_kernel(ViewParams vp, BlendArgs a) {
+__global__ __launch_bounds__(64) void blend_kernel(const ViewParams* vpp, BlendArgs a) {
+    const ViewParams vp = *vpp;
     __shared__ float4 sV1[64], sV2[64], sV4[64], sC[64];
     const uint32_t g = xcd_remap(blockIdx.x, gridDim.x, a.xcd_run);
     const uint32_t* fi = a.frame_info;
@@ -295,9 +296,12 @@ __global__ __launch_bounds__(64) void blend_kernel(ViewParams vp, BlendArgs a) {
     }
 }
 
-hipError_t launch_blend(const ViewParams& vp, const ViewBuffers& vb, const uint32_t* sorted_vals,
-                        int K, uint32_t n_active, int xcd_run, hipStream_t s) {
-    if (n_active == 0) return hipSuccess;
+// `grid` is the active tile count, or n_small as a bound: workgroups outside
+// the three partition ranges return at once.
+hipError_t launch_blend(const ViewParams* dvp, const ViewBuffers& vb, const uint32_t* sorted_vals,
+                        int K, uint32_t grid, int xcd_run, hipStream_t s) {
+    if (grid == 0) return hipSuccess;
+    const uint32_t n_active = grid;
     BlendArgs a;
     a.frame_info = vb.frame_info;
     a.tile_map_part = vb.tile_map_part;
@@ -306,12 +310,12 @@ hipError_t launch_blend(const ViewParams& vp, const ViewBuffers& vb, const uint3
     a.prim = vb.prim;
     a.xcd_run = (uint32_t)(xcd_run > 0 ? xcd_run : 0);
     // rasterization/src/inference.cu:17-31: largest power of two <= K in 1..32
-    if (K >= 32) blend_kernel<32><<<n_active, 64, 0, s>>>(vp, a);
-    else if (K >= 16) blend_kernel<16><<<n_active, 64, 0, s>>>(vp, a);
-    else if (K >= 8) blend_kernel<8><<<n_active, 64, 0, s>>>(vp, a);
-    else if (K >= 4) blend_kernel<4><<<n_active, 64, 0, s>>>(vp, a);
-    else if (K >= 2) blend_kernel<2><<<n_active, 64, 0, s>>>(vp, a);
-    else blend_kernel<1><<<n_active, 64, 0, s>>>(vp, a);
+    if (K >= 32) blend_kernel<32><<<n_active, 64, 0, s>>>(dvp, a);
+    else if (K >= 16) blend_kernel<16><<<n_active, 64, 0, s>>>(dvp, a);
+    else if (K >= 8) blend_kernel<8><<<n_active, 64, 0, s>>>(dvp, a);
+    else if (K >= 4) blend_kernel<4><<<n_active, 64, 0, s>>>(dvp, a);
+    else if (K >= 2) blend_kernel<2><<<n_active, 64, 0, s>>>(dvp, a);
+    else blend_kernel<1><<<n_active, 64, 0, s>>>(dvp, a);
     return hipGetLastError();
 }
 

@@ -52,7 +52,8 @@ constexpr int kTilesThreads = 1024;
 constexpr int kTilesMaxPer = 64;  // tiles per thread: up to 65536 large tiles
 constexpr int kTilesMaskWords = kTilesThreads * kTilesMaxPer / 32;
 
-__global__ __launch_bounds__(kTilesThreads) void tiles_kernel(ViewParams vp, ViewBuffers vb) {
+__global__ __launch_bounds__(kTilesThreads) void tiles_kernel(const ViewParams* vpp, ViewBuffers vb) {
+    const ViewParams vp = *vpp;
     __shared__ uint32_t s_mask[kTilesMaskWords];
     __shared__ uint32_t s_code[kTilesMaskWords * 2];  // 2 bits per tile
     const int n = vp.n_large;
@@ -119,9 +120,9 @@ __global__ __launch_bounds__(kTilesThreads) void tiles_kernel(ViewParams vp, Vie
     }
 }
 
-hipError_t launch_tiles(const ViewParams& vp, const ViewBuffers& vb, hipStream_t s) {
+hipError_t launch_tiles(const ViewParams& vp, const ViewParams* dvp, const ViewBuffers& vb, hipStream_t s) {
     if (vp.n_large > kTilesThreads * kTilesMaxPer) return hipErrorInvalidValue;
-    tiles_kernel<<<1, kTilesThreads, 0, s>>>(vp, vb);
+    tiles_kernel<<<1, kTilesThreads, 0, s>>>(dvp, vb);
     return hipGetLastError();
 }
 
@@ -161,9 +162,12 @@ hipError_t launch_scan_blocks(const ViewBuffers& vb, uint32_t n_blocks, hipStrea
 // Render-path output clear (inference_api.cu:79-83 allocates zeros): only the
 // pixels no kernel writes are cleared -- invisible large tiles in the final
 // image, and, when blurring, invisible and FOVEA tiles of the temp image,
-// which the blur reads. With clear_all (no tile rendered) everything is.
-__global__ __launch_bounds__(256) void clear_image_kernel(ViewParams vp, const uint32_t* tile_num,
-                                                          int clear_all) {
+// which the blur reads. When at most one tile is active (the partition ranges
+// are then empty, so nothing is rendered) everything is.
+__global__ __launch_bounds__(256) void clear_image_kernel(const ViewParams* vpp, const uint32_t* tile_num,
+                                                          const uint32_t* frame_info) {
+    const ViewParams vp = *vpp;
+    const bool clear_all = frame_info[FI_N_ACTIVE] <= 1u;
     const uint32_t t = blockIdx.x;
     const int lx = (int)(t % (uint32_t)vp.gw), ly = (int)(t / (uint32_t)vp.gw);
     const int px = lx * kTileL + (int)(threadIdx.x & 15u);
@@ -196,10 +200,10 @@ __global__ __launch_bounds__(256) void clear_image_kernel(ViewParams vp, const u
     }
 }
 
-hipError_t launch_clear_image(const ViewParams& vp, const ViewBuffers& vb, int clear_all,
+hipError_t launch_clear_image(const ViewParams& vp, const ViewParams* dvp, const ViewBuffers& vb,
                               hipStream_t s) {
     if (vp.n_large <= 0) return hipSuccess;
-    clear_image_kernel<<<vp.n_large, 256, 0, s>>>(vp, vb.tile_num, clear_all);
+    clear_image_kernel<<<vp.n_large, 256, 0, s>>>(dvp, vb.tile_num, vb.frame_info);
     return hipGetLastError();
 }
 

@@ -15,7 +15,16 @@ namespace fhtgs {
 constexpr int kMaxViewsPerPass = 2;
 constexpr int kShChunk = 8;  // visible Gaussians per SH gather round
 
+// Kernel argument: device pointers to the views' parameters and the scene.
 struct PreprocessArgs {
+    const ViewParams* vp[kMaxViewsPerPass];
+    ViewBuffers vb[kMaxViewsPerPass];
+    const SceneArgs* scene;
+    uint32_t n;
+};
+
+// The same, resolved into registers at kernel start.
+struct PreprocessState {
     ViewParams vp[kMaxViewsPerPass];
     ViewBuffers vb[kMaxViewsPerPass];
     const float* pos;
@@ -26,6 +35,20 @@ struct PreprocessArgs {
     const float* shr;
     uint32_t n;
 };
+
+template <int NV>
+__device__ __forceinline__ PreprocessState resolve(const PreprocessArgs& g) {
+    PreprocessState a;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        a.vp[v] = *g.vp[v];
+        a.vb[v] = g.vb[v];
+    }
+    const SceneArgs sc = *g.scene;
+    a.pos = sc.pos; a.scl = sc.scl; a.rot = sc.rot; a.opa = sc.opa; a.sh0 = sc.sh0; a.shr = sc.shr;
+    a.n = g.n;
+    return a;
+}
 
 // sh_utils.cuh:9-30
 constexpr float C0 = 0.28209479177387814f;
@@ -274,7 +297,8 @@ __device__ __forceinline__ uint32_t tiles_touched(const ViewParams& vp, uint32_t
 }
 
 template <int NV>
-__global__ __launch_bounds__(kPrimBlock) void preprocess_aaa_kernel(PreprocessArgs a) {
+__global__ __launch_bounds__(kPrimBlock) void preprocess_aaa_kernel(PreprocessArgs g) {
+    const PreprocessState a = resolve<NV>(g);
     __shared__ uint32_t s_sum[NV][kPrimBlock / 64];
     __shared__ uint32_t s_vis[NV][kPrimBlock / 64];
     const uint32_t i = blockIdx.x * kPrimBlock + threadIdx.x;
@@ -326,8 +350,10 @@ __global__ __launch_bounds__(kPrimBlock) void preprocess_aaa_kernel(PreprocessAr
         if (lane == 0) { s_sum[v][wid] = ws; s_vis[v][wid] = wv; }
     }
     __syncthreads();
-    if (threadIdx.x < NV) {
-        const int v = threadIdx.x;
+    // thread v finishes view v (static indices keep the parameters in registers)
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        if ((int)threadIdx.x != v) continue;
         uint32_t tot = 0, vis = 0;
         for (int w = 0; w < kPrimBlock / 64; ++w) { tot += s_sum[v][w]; vis += s_vis[v][w]; }
         a.vb[v].block_sums[blockIdx.x] = tot;
@@ -336,7 +362,8 @@ __global__ __launch_bounds__(kPrimBlock) void preprocess_aaa_kernel(PreprocessAr
 }
 
 template <int NV>
-__global__ __launch_bounds__(kPrimBlock) void preprocess_kernel(PreprocessArgs a) {
+__global__ __launch_bounds__(kPrimBlock) void preprocess_kernel(PreprocessArgs g) {
+    const PreprocessState a = resolve<NV>(g);
     extern __shared__ float s_sh[];  // per wave: 64 slots of the first min(3S, 45) SH floats
     __shared__ uint32_t s_sum[NV][kPrimBlock / 64];
     __shared__ uint32_t s_vis[NV][kPrimBlock / 64];
@@ -492,8 +519,10 @@ __global__ __launch_bounds__(kPrimBlock) void preprocess_kernel(PreprocessArgs a
         if (lane == 0) { s_sum[v][wid] = ws; s_vis[v][wid] = wv; }
     }
     __syncthreads();
-    if (threadIdx.x < NV) {
-        const int v = threadIdx.x;
+    // thread v finishes view v (static indices keep the parameters in registers)
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        if ((int)threadIdx.x != v) continue;
         uint32_t tot = 0, vis = 0;
         for (int w = 0; w < kPrimBlock / 64; ++w) { tot += s_sum[v][w]; vis += s_vis[v][w]; }
         a.vb[v].block_sums[blockIdx.x] = tot;
@@ -501,22 +530,26 @@ __global__ __launch_bounds__(kPrimBlock) void preprocess_kernel(PreprocessArgs a
     }
 }
 
-hipError_t launch_preprocess(const ViewParams* vps, const ViewBuffers* vbs, int n_views,
-                             const float* pos, const float* scl, const float* rot,
-                             const float* opa, const float* sh0, const float* shr, uint32_t n,
-                             int, hipStream_t s) {
+hipError_t launch_preprocess(const ViewParams* const* vps, const ViewParams* const* dvps,
+                             const ViewBuffers* vbs, int n_views, const SceneArgs* dscene,
+                             uint32_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
     if (n_views < 1 || n_views > kMaxViewsPerPass) return hipErrorInvalidValue;
     PreprocessArgs a;
-    for (int v = 0; v < n_views; ++v) { a.vp[v] = vps[v]; a.vb[v] = vbs[v]; }
-    a.pos = pos; a.scl = scl; a.rot = rot; a.opa = opa; a.sh0 = sh0; a.shr = shr; a.n = n;
+    for (int v = 0; v < kMaxViewsPerPass; ++v) {
+        const int k = v < n_views ? v : 0;
+        a.vp[v] = dvps[k];
+        a.vb[v] = vbs[k];
+    }
+    a.scene = dscene;
+    a.n = n;
     const uint32_t blocks = (n + kPrimBlock - 1) / kPrimBlock;
     // inference.cuh:86-112: the anti-aliased variant replaces transform and cull
-    if (vps[0].aaa) {
+    if (vps[0]->aaa) {
         if (n_views == 1) preprocess_aaa_kernel<1><<<blocks, kPrimBlock, 0, s>>>(a);
         else preprocess_aaa_kernel<2><<<blocks, kPrimBlock, 0, s>>>(a);
     } else {
-        const size_t lds = sizeof(float) * (size_t)(kPrimBlock / 64) * kShChunk * (size_t)min(3 * vps[0].total_sh, 45);
+        const size_t lds = sizeof(float) * (size_t)(kPrimBlock / 64) * kShChunk * (size_t)min(3 * vps[0]->total_sh, 45);
         if (n_views == 1) preprocess_kernel<1><<<blocks, kPrimBlock, lds, s>>>(a);
         else preprocess_kernel<2><<<blocks, kPrimBlock, lds, s>>>(a);
     }

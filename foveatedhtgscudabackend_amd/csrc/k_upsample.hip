@@ -111,8 +111,10 @@ __device__ __forceinline__ float ring_factor(const ViewParams& vp, int px, int p
 
 __device__ __forceinline__ v3 sat3(v3 c) { return v3{fh_saturate(c.x), fh_saturate(c.y), fh_saturate(c.z)}; }
 
-__global__ __launch_bounds__(256) void upsample_kernel(ViewParams vp, const uint32_t* frame_info,
-                                                       const uint32_t* tile_map_part, uint32_t n_periphery) {
+__global__ __launch_bounds__(256) void upsample_kernel(const ViewParams* vpp, const uint32_t* frame_info,
+                                                       const uint32_t* tile_map_part) {
+    const ViewParams vp = *vpp;
+    const uint32_t n_periphery = frame_info[FI_PART + 3] - frame_info[FI_PART + 2];
     __shared__ v3 s_smp[4 * 36 > 100 ? 4 * 36 : 100];
     const int tid = threadIdx.x;
     const int W = vp.width, H = vp.height;
@@ -192,11 +194,12 @@ __global__ __launch_bounds__(256) void upsample_kernel(ViewParams vp, const uint
     }
 }
 
-hipError_t launch_upsample(const ViewParams& vp, const ViewBuffers& vb, uint32_t n_periphery,
-                           uint32_t n_blended, hipStream_t s) {
-    const uint32_t blocks = n_periphery + (n_blended + 3u) / 4u;
-    if (blocks == 0) return hipSuccess;
-    upsample_kernel<<<blocks, 256, 0, s>>>(vp, vb.frame_info, vb.tile_map_part, n_periphery);
+// One workgroup per PERIPHERY large tile, then one per four BLENDED small
+// tiles (a blended large tile's four); `grid` is that count, or n_large as a
+// bound -- workgroups past the partition find no tile and return.
+hipError_t launch_upsample(const ViewParams* dvp, const ViewBuffers& vb, uint32_t grid, hipStream_t s) {
+    if (grid == 0) return hipSuccess;
+    upsample_kernel<<<grid, 256, 0, s>>>(dvp, vb.frame_info, vb.tile_map_part);
     return hipGetLastError();
 }
 

@@ -19,6 +19,9 @@
  *     shared preprocess)         rasterization/include/inference_stereo_api.h:7-61,
  *                                rasterization/src/inference_stereo_api.cu:13-192
  *                                (_C_stereo.render, bindings_stereo.cpp:8)
+ *   fhtgs_render_batch        <- no reference counterpart: many views of several
+ *                                resident scenes per call (the reference renders one
+ *                                view or one stereo pair per call)
  *   fhtgs_workspace_bytes     <- the per-call scratch sizing of required<>/from_blob,
  *                                rasterization/include/utils/buffer_utils.h:10-143
  *   fhtgs_sort_pairs          <- cub::DeviceRadixSort::SortPairs over bits [0,end_bit),
@@ -114,14 +117,22 @@ const char* fhtgs_last_error(fhtgs_handle h);
 const char* fhtgs_version(void);
 
 /* Renders n_views views of one scene, enqueued on `stream`. With
- * share_preprocess != 0 the Gaussians are read once for all views (stereo);
- * the views must then agree on K, active_sh_bases, near/far, scale_modifier
- * and anti_aliasing (the reference stereo path uses the left eye's values).
- * The call waits on the stream twice: for the instance and tile counts after
- * the preprocess, and at the end (the reference call also returns only after
- * its streams finish, rasterization/src/inference.cu:468-470). */
+ * share_preprocess != 0 the Gaussians are read once for both views (stereo,
+ * n_views <= 2); the views must then agree on active_sh_bases, near/far,
+ * scale_modifier and anti_aliasing (the reference stereo path uses the left
+ * eye's values). The call returns after the frame is finished (as the
+ * reference call does, rasterization/src/inference.cu:468-470). How it waits
+ * depends on FHTGS_OPT_FRAME_MODE; the images are the same in every mode. */
 int fhtgs_render(fhtgs_handle h, const fhtgs_scene* scene, const fhtgs_view* views,
                  int32_t n_views, int32_t share_preprocess, void* stream);
+
+/* Renders n_views views of n_scenes scenes in one call: view v shows
+ * scenes[view_scene[v]]. Consecutive views of one scene that agree on the
+ * preprocess parameters share one preprocess pass (stereo pairs); each
+ * view's image equals what fhtgs_render gives for it alone. */
+int fhtgs_render_batch(fhtgs_handle h, const fhtgs_scene* scenes, int32_t n_scenes,
+                       const fhtgs_view* views, const int32_t* view_scene, int32_t n_views,
+                       void* stream);
 
 /* Device scratch the handle currently holds (it grows on demand). */
 int fhtgs_workspace_bytes(fhtgs_handle h, int64_t* bytes);
@@ -171,6 +182,16 @@ int fhtgs_debug_counts(fhtgs_handle h, int32_t view, int64_t* counts);
  * one XCD (its own L2) so neighbouring tiles reuse primitive records; 0 or 1
  * keeps the launch order. Values 0..4096, default 16. Images do not depend on it. */
 #define FHTGS_OPT_BLEND_XCD_RUN 4
+/* How a frame waits for the instance count that sizes its sort:
+ *   0  exact: read the counts back after the preprocess (one mid-frame wait),
+ *      size the instance buffers to them; the reference's behaviour
+ *   1  device counts: no mid-frame wait; kernels read the counts on the GPU
+ *      and the buffers hold the last exact frame's count plus headroom
+ *   2  as 1, replayed from a captured hipGraph while the launch shape (views,
+ *      sizes, K, scenes' sizes, buffers) stays the same (default)
+ * A frame that needs more instances than its buffers hold in mode 1/2 is
+ * rendered again in mode 0; the first frame of a new shape is always exact. */
+#define FHTGS_OPT_FRAME_MODE 5
 int fhtgs_set_option(fhtgs_handle h, int32_t option, int64_t value);
 
 /* ms per stage of the last timed frame, all views of the call together:
