@@ -50,6 +50,44 @@ __device__ __forceinline__ float rcp_rn(float x) {
     return 1.0f / x;
 }
 
+// Pointers read from the device parameter block are generic to the
+// compiler, which then emits flat loads and stores through them. Reading
+// each pointer field through a global-address-space view of its storage
+// makes the loaded value a global pointer, and the accesses global ones.
+// Every such pointer is a hipMalloc'd device address.
+template <typename T>
+__device__ __forceinline__ T* load_gptr(T* const* field) {
+    typedef __attribute__((address_space(1))) T GT;
+    return (T*)(*(GT* const*)field);
+}
+
+// Kernel-side copy of a view's parameters with global pointers.
+__device__ __forceinline__ ViewParams load_view(const ViewParams* src) {
+    ViewParams v = *src;
+    v.M = load_gptr(&src->M);
+    v.VPM = load_gptr(&src->VPM);
+    v.VPR_inv = load_gptr(&src->VPR_inv);
+    v.cam = load_gptr(&src->cam);
+    v.vis_mask = load_gptr(&src->vis_mask);
+    v.vis_sat = load_gptr(&src->vis_sat);
+    v.fovea_sat = load_gptr(&src->fovea_sat);
+    v.bg_data = load_gptr(&src->bg_data);
+    v.image_final = load_gptr(&src->image_final);
+    v.image_blend = load_gptr(&src->image_blend);
+    return v;
+}
+
+__device__ __forceinline__ SceneArgs load_scene(const SceneArgs* src) {
+    SceneArgs s;
+    s.pos = load_gptr(&src->pos);
+    s.scl = load_gptr(&src->scl);
+    s.rot = load_gptr(&src->rot);
+    s.opa = load_gptr(&src->opa);
+    s.sh0 = load_gptr(&src->sh0);
+    s.shr = load_gptr(&src->shr);
+    return s;
+}
+
 __device__ __forceinline__ v4 load_row(const float* m, int row) {
     return v4{m[4 * row + 0], m[4 * row + 1], m[4 * row + 2], m[4 * row + 3]};
 }

@@ -135,7 +135,7 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n, uint32_t r
 
 template <int K>
 __global__ __launch_bounds__(64) void blend_kernel(const ViewParams* vpp, BlendArgs a) {
-    const ViewParams vp = *vpp;
+    const ViewParams vp = load_view(vpp);
     __shared__ float4 sV1[64], sV2[64], sV4[64], sC[64];
     const uint32_t g = xcd_remap(blockIdx.x, gridDim.x, a.xcd_run);
     const uint32_t* fi = a.frame_info;

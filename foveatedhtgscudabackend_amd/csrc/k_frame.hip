@@ -53,7 +53,7 @@ constexpr int kTilesMaxPer = 64;  // tiles per thread: up to 65536 large tiles
 constexpr int kTilesMaskWords = kTilesThreads * kTilesMaxPer / 32;
 
 __global__ __launch_bounds__(kTilesThreads) void tiles_kernel(const ViewParams* vpp, ViewBuffers vb) {
-    const ViewParams vp = *vpp;
+    const ViewParams vp = load_view(vpp);
     __shared__ uint32_t s_mask[kTilesMaskWords];
     __shared__ uint32_t s_code[kTilesMaskWords * 2];  // 2 bits per tile
     const int n = vp.n_large;
@@ -166,7 +166,7 @@ hipError_t launch_scan_blocks(const ViewBuffers& vb, uint32_t n_blocks, hipStrea
 // are then empty, so nothing is rendered) everything is.
 __global__ __launch_bounds__(256) void clear_image_kernel(const ViewParams* vpp, const uint32_t* tile_num,
                                                           const uint32_t* frame_info) {
-    const ViewParams vp = *vpp;
+    const ViewParams vp = load_view(vpp);
     const bool clear_all = frame_info[FI_N_ACTIVE] <= 1u;
     const uint32_t t = blockIdx.x;
     const int lx = (int)(t % (uint32_t)vp.gw), ly = (int)(t / (uint32_t)vp.gw);

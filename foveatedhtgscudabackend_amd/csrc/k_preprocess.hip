@@ -41,10 +41,10 @@ __device__ __forceinline__ PreprocessState resolve(const PreprocessArgs& g) {
     PreprocessState a;
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
-        a.vp[v] = *g.vp[v];
+        a.vp[v] = load_view(g.vp[v]);
         a.vb[v] = g.vb[v];
     }
-    const SceneArgs sc = *g.scene;
+    const SceneArgs sc = load_scene(g.scene);
     a.pos = sc.pos; a.scl = sc.scl; a.rot = sc.rot; a.opa = sc.opa; a.sh0 = sc.sh0; a.shr = sc.shr;
     a.n = g.n;
     return a;

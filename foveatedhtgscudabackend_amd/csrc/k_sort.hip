@@ -54,7 +54,7 @@ __global__ __launch_bounds__(kPrimBlock) void instances_kernel(const ViewParams*
         if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&vb.frame_info[FI_ERROR], 4u);
         return;
     }
-    const ViewParams vp = *vpp;
+    const ViewParams vp = load_view(vpp);
     extern __shared__ uint32_t s_hist[];
     __shared__ uint32_t s_wave[kPrimBlock / 64];
     __shared__ uint32_t s_valid[kPrimBlock / 64];

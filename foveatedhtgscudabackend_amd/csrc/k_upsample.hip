@@ -113,7 +113,7 @@ __device__ __forceinline__ v3 sat3(v3 c) { return v3{fh_saturate(c.x), fh_satura
 
 __global__ __launch_bounds__(256) void upsample_kernel(const ViewParams* vpp, const uint32_t* frame_info,
                                                        const uint32_t* tile_map_part) {
-    const ViewParams vp = *vpp;
+    const ViewParams vp = load_view(vpp);
     const uint32_t n_periphery = frame_info[FI_PART + 3] - frame_info[FI_PART + 2];
     __shared__ v3 s_smp[4 * 36 > 100 ? 4 * 36 : 100];
     const int tid = threadIdx.x;
