@@ -87,12 +87,24 @@ def main() -> int:
     ap.add_argument("--cpu-tile-stride", type=int, default=4)
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run 3 plain device copies of 256 MiB (calibrates FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--workload", choices=("stereo", "batched", "serving"), default="stereo",
+                    help="stereo: one stereo foveated frame (default line); batched: 24 mixed view "
+                         "requests over 4 resident scenes; serving: 8 headsets at 90 Hz over 8 scenes")
+    ap.add_argument("--duration", type=float, default=2.0, help="serving: seconds of requests")
+    ap.add_argument("--serve-batch", type=int, default=1, help="serving: most headsets per batch")
+    ap.add_argument("--serve-workers", type=int, default=3, help="serving: batches in flight")
+    ap.add_argument("--headsets", type=int, default=8, help="serving: headsets (at most 8, one scene each)")
+    ap.add_argument("--serve-graph", type=int, default=1, help="serving: replay captured graphs (1) or not (0)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the renderer shards by view/viewer with no collectives, "
               "and this build runs on a single MI355X only (--gpus 1).", file=sys.stderr)
         return 2
+    if args.workload == "batched":
+        return bench_batched(args)
+    if args.workload == "serving":
+        return bench_serving(args)
 
     import torch
     from foveatedhtgscudabackend_amd import FoveatedHTGSRasterizer, _lib
@@ -246,6 +258,129 @@ def main() -> int:
     }
     print(json.dumps(out))
     return 0
+
+
+def _common(args, metric, value, unit, ms, config, extra):
+    out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded generator, inputs resident in HBM)",
+           "config": config}
+    out.update(extra)
+    print(json.dumps(out))
+    return 0
+
+
+def bench_batched(args) -> int:
+    """24 view requests of uneven size (640x480 .. 2064x2208), half of them
+    stereo pairs, half foveated, over 4 resident scenes (0.5/1.5/3/6 M
+    Gaussians), as one autotuned batch replayed from a captured graph,
+    against the plain path (one view per call, exact frames)."""
+    import torch
+    from foveatedhtgscudabackend_amd import FoveatedHTGSRasterizer, _lib
+    from foveatedhtgscudabackend_amd import scene as S
+    from foveatedhtgscudabackend_amd.batch import BatchView, autotune, render_batch
+    rng = np.random.default_rng(args.seed + 4)
+    sizes_m = (500_000, 1_500_000, 3_000_000, 6_000_000)
+    scenes = [S.scene_to_torch(S.make_scene(n, seed=args.seed + 10 + k, cube=args.cube), "cuda")
+              for k, n in enumerate(sizes_m)]
+    views = []
+    for r in range(24):
+        w = int(rng.integers(640, 2065)) // 16 * 16
+        h = int(rng.integers(480, 2209)) // 16 * 16
+        sc = int(rng.integers(0, 4))
+        fov = float(rng.uniform(60.0, 100.0))
+        fr = 375 if rng.random() < 0.5 else -1
+        if rng.random() < 0.5:
+            cams = S.stereo_cameras(w, h, fov, seed=args.seed + 3000 + r)
+        else:
+            cams = (S.orbit_camera(w, h, fov, seed=args.seed + 3000 + r),)
+        gaze = S.seeded_gaze(cams[0], args.seed + 5000 + r)
+        for cam in cams:
+            st = S.settings_for(cam, gaze, K=args.K, foveation_radius_px=fr)
+            views.append(BatchView(sc, st, foveation_radius_px=fr, clear_output=False))
+    handle = _lib.handle_for(torch.cuda.current_device())
+    t0 = time.perf_counter()
+    tuned = autotune(scenes, views, frames=2)
+    tune_s = time.perf_counter() - t0
+    handle.set_option(_lib.OPT_FRAME_MODE, _lib.FRAME_GRAPH)
+    for _ in range(args.warmup):
+        render_batch(scenes, views)
+    torch.cuda.synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s0 = time.perf_counter()
+        render_batch(scenes, views)
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - s0) * 1e3)
+    total = time.perf_counter() - t0
+    n_views = len(views)
+    plain_vps = None
+    if not args.no_plain:
+        rast = {}
+        handle.set_option(_lib.OPT_FRAME_MODE, _lib.FRAME_EXACT)
+
+        def plain():
+            for bv in views:
+                r = rast.setdefault(bv.foveation_radius_px, FoveatedHTGSRasterizer(bv.foveation_radius_px, 125))
+                r.benchmark(*scenes[bv.scene], bv.settings, False, False, False)
+        plain()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(max(3, args.steps // 4)):
+            plain()
+            torch.cuda.synchronize()
+        plain_vps = max(3, args.steps // 4) * n_views / (time.perf_counter() - t1)
+        handle.set_option(_lib.OPT_FRAME_MODE, _lib.FRAME_GRAPH)
+    vps = args.steps * n_views / total
+    cfg = {"workload": "batched: 24 view requests (640x480..2064x2208, half stereo pairs, half foveated 375/125 px, "
+                       "K=8) over 4 resident scenes of 0.5/1.5/3/6 M Gaussians, one autotuned batch per step, "
+                       "graph replay", "views_per_step": n_views, "scenes_m": [n / 1e6 for n in sizes_m],
+           "parallelism": "single GPU"}
+    return _common(args, "batched views/sec, 1 GPU (vs one view per call)", vps, "views/s", 1e3 * total / args.steps,
+                   cfg, {"p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99), "plain_views_per_s": plain_vps,
+                         "batched_vs_plain": (vps / plain_vps) if plain_vps else None, "autotune": tuned,
+                         "autotune_s": round(tune_s, 2), "roofline": None, "cpu_baseline": None})
+
+
+def bench_serving(args) -> int:
+    """8 headsets requesting stereo foveated 2064x2208 frames at 90 Hz over 8
+    resident scenes of 1..6 M Gaussians (24 M), continuous batching on two
+    workers, against the plain per-request path on the same arrivals."""
+    import torch
+    from foveatedhtgscudabackend_amd import scene as S
+    from foveatedhtgscudabackend_amd import serving as V
+    sizes = (1_000_000, 1_500_000, 2_000_000, 2_500_000, 3_000_000, 3_500_000, 4_500_000, 6_000_000)
+    scenes = [S.scene_to_torch(S.make_scene(n, seed=args.seed + 20 + k, cube=args.cube), "cuda")
+              for k, n in enumerate(sizes)]
+    headsets = [V.HeadsetSim(h, h, seed=args.seed + 700 + h) for h in range(max(1, min(8, args.headsets)))]
+    reqs = V.make_requests(headsets, args.duration, args.width, args.height, args.fov)
+    srv = V.Server(scenes, workers=args.serve_workers, max_batch_headsets=args.serve_batch,
+                   graph=bool(args.serve_graph))
+    admitted = [hs.hid for hs in headsets if srv.admit(hs.hid, hs.scene, args.width, args.height)]
+    reqs = [r for r in reqs if r.headset in srv.admitted]
+    # warm-up: one frame per headset sizes every slot (exact) before the clock
+    warm = [V.Request(r.headset, -1, 0.0, r.settings) for r in
+            {r.headset: r for r in reversed(reqs)}.values()]
+    V.Server.run(srv, warm)
+    srv.done, srv.closed = [], False
+    stats = srv.run(reqs)
+    plain = None
+    if not args.no_plain:
+        for r in reqs:
+            r.dropped, r.images = False, None
+        plain = V.run_plain(scenes, srv.admitted, reqs)
+    cfg = {"workload": f"serving: {len(headsets)} headsets x stereo 2064x2208 @ 90 Hz (jitter 1 ms), gaze random walk with "
+                       f"saccades, 8 resident scenes 1..6 M ({sum(sizes) / 1e6:.0f} M Gaussians), "
+                       f"{args.serve_workers} workers, batches of up to {args.serve_batch} headsets, latest frame wins",
+           "duration_s": args.duration, "admitted": admitted, "graph": bool(args.serve_graph),
+           "rejected": srv.rejected, "parallelism": "single GPU"}
+    return _common(args, "served stereo foveated frames/sec + p50/p99 latency, 1 GPU", stats["frames_per_s"],
+                   "stereo frames/s", stats["wall_s"] * 1e3 / max(stats["served"], 1), cfg,
+                   {"p50_ms": stats["p50_ms"], "p99_ms": stats["p99_ms"], "served": stats["served"],
+                    "dropped": stats["dropped"], "requests": stats["requests"], "plain": plain,
+                    "served_vs_plain": (stats["frames_per_s"] / plain["frames_per_s"]) if plain else None,
+                    "roofline": None, "cpu_baseline": None})
 
 
 if __name__ == "__main__":

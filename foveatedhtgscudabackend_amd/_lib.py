@@ -44,7 +44,7 @@ EXPORTED_SYMBOLS = (
     "fhtgs_create", "fhtgs_destroy", "fhtgs_last_error", "fhtgs_version", "fhtgs_render",
     "fhtgs_workspace_bytes", "fhtgs_sort_pairs", "fhtgs_fovea_radius_tiles",
     "fhtgs_debug_read", "fhtgs_debug_counts", "fhtgs_set_option", "fhtgs_stage_times",
-    "fhtgs_render_batch",
+    "fhtgs_render_batch", "fhtgs_autotune", "fhtgs_tuned_digit_bits", "fhtgs_get_option",
 )
 
 
@@ -107,8 +107,15 @@ def load() -> ctypes.CDLL:
         lib.fhtgs_render.argtypes = [vp, ctypes.POINTER(Scene), ctypes.POINTER(View), i32, i32, vp]
         lib.fhtgs_render.restype = ctypes.c_int
         lib.fhtgs_render_batch.argtypes = [vp, ctypes.POINTER(Scene), i32, ctypes.POINTER(View),
-                                           ctypes.POINTER(i32), i32, vp]
+                                           ctypes.POINTER(i32), ctypes.POINTER(i32), i32, vp]
         lib.fhtgs_render_batch.restype = ctypes.c_int
+        lib.fhtgs_autotune.argtypes = [vp, ctypes.POINTER(Scene), i32, ctypes.POINTER(View),
+                                       ctypes.POINTER(i32), ctypes.POINTER(i32), i32, i32, vp]
+        lib.fhtgs_autotune.restype = ctypes.c_int
+        lib.fhtgs_tuned_digit_bits.argtypes = [vp, i32, i32, ctypes.POINTER(i32)]
+        lib.fhtgs_tuned_digit_bits.restype = ctypes.c_int
+        lib.fhtgs_get_option.argtypes = [vp, i32, ctypes.POINTER(i64)]
+        lib.fhtgs_get_option.restype = ctypes.c_int
         lib.fhtgs_workspace_bytes.argtypes = [vp, ctypes.POINTER(i64)]
         lib.fhtgs_workspace_bytes.restype = ctypes.c_int
         lib.fhtgs_sort_pairs.argtypes = [vp, vp, vp, i64, i32, i32, vp]
@@ -169,14 +176,40 @@ class Handle:
                                        1 if share_preprocess else 0, ctypes.c_void_p(stream))
             self.check(rc, "fhtgs_render")
 
-    def render_batch(self, scenes: list, views: list, view_scene: list, stream: int) -> None:
+    @staticmethod
+    def _batch_arrays(scenes, views, view_scene, view_slot):
         sarr = (Scene * len(scenes))(*scenes)
         varr = (View * len(views))(*views)
         iarr = (ctypes.c_int32 * len(views))(*[int(k) for k in view_scene])
+        slots = None if view_slot is None else (ctypes.c_int32 * len(views))(*[int(k) for k in view_slot])
+        return sarr, varr, iarr, slots
+
+    def render_batch(self, scenes: list, views: list, view_scene: list, stream: int,
+                     view_slot: list | None = None) -> None:
+        sarr, varr, iarr, slots = self._batch_arrays(scenes, views, view_scene, view_slot)
         with self.lock:
-            rc = self.lib.fhtgs_render_batch(self.ptr, sarr, len(scenes), varr, iarr, len(views),
+            rc = self.lib.fhtgs_render_batch(self.ptr, sarr, len(scenes), varr, iarr, slots, len(views),
                                              ctypes.c_void_p(stream))
             self.check(rc, "fhtgs_render_batch")
+
+    def autotune(self, scenes: list, views: list, view_scene: list, stream: int, frames: int = 3,
+                 view_slot: list | None = None) -> None:
+        sarr, varr, iarr, slots = self._batch_arrays(scenes, views, view_scene, view_slot)
+        with self.lock:
+            rc = self.lib.fhtgs_autotune(self.ptr, sarr, len(scenes), varr, iarr, slots, len(views),
+                                         int(frames), ctypes.c_void_p(stream))
+            self.check(rc, "fhtgs_autotune")
+
+    def get_option(self, option: int) -> int:
+        v = ctypes.c_int64()
+        self.check(self.lib.fhtgs_get_option(self.ptr, option, ctypes.byref(v)), "fhtgs_get_option")
+        return v.value
+
+    def tuned_digit_bits(self, width: int, height: int) -> int:
+        b = ctypes.c_int32()
+        self.check(self.lib.fhtgs_tuned_digit_bits(self.ptr, int(width), int(height), ctypes.byref(b)),
+                   "fhtgs_tuned_digit_bits")
+        return b.value
 
     def sort_pairs(self, keys_ptr: int, vals_ptr: int, n: int, key_bytes: int, end_bit: int,
                    stream: int) -> None:

@@ -29,14 +29,12 @@ class BatchView:
     foveation_radius_px: int = _lib.DEFAULT_FOVEATION_RADIUS_PX
     blend_width_px: int = _lib.DEFAULT_BLEND_WIDTH_PX
     clear_output: bool = True
+    slot: int | None = None  # handle slot (persistent buffers), default: position in the batch
 
 
-def render_batch(scenes: list, views: list) -> list:
-    """scenes: list of (positions, scales, rotations, opacities, sh_0, sh_rest)
-    tuples on one GPU; views: list of BatchView. Returns one image per view
-    ((H,W,3) or (3,H,W) float32), rendered on the current stream."""
+def _marshal(scenes: list, views: list):
     if not scenes or not views:
-        raise ValueError("render_batch needs at least one scene and one view")
+        raise ValueError("a batch needs at least one scene and one view")
     keep: list = []
     structs, device = [], None
     for sc in scenes:
@@ -55,8 +53,31 @@ def render_batch(scenes: list, views: list) -> list:
         vstructs.append(v)
         images.append(img)
         index.append(bv.scene)
+    given = [bv.slot is not None for bv in views]
+    if any(given) and not all(given):
+        raise ValueError("give a slot for every view of the batch or for none")
+    slots = [bv.slot for bv in views] if all(given) else None
     handle = _lib.handle_for(device.index if device.index is not None else torch.cuda.current_device())
     stream = torch.cuda.current_stream(device).cuda_stream
-    handle.render_batch(structs, vstructs, index, stream)
+    return handle, structs, vstructs, index, slots, images, stream, keep
+
+
+def render_batch(scenes: list, views: list) -> list:
+    """scenes: list of (positions, scales, rotations, opacities, sh_0, sh_rest)
+    tuples on one GPU; views: list of BatchView. Returns one image per view
+    ((H,W,3) or (3,H,W) float32), rendered on the current stream."""
+    handle, structs, vstructs, index, slots, images, stream, keep = _marshal(scenes, views)
+    handle.render_batch(structs, vstructs, index, stream, slots)
     del keep
     return images
+
+
+def autotune(scenes: list, views: list, frames: int = 3) -> dict:
+    """Tunes the handle for this batch (sort digit width per view size, blend
+    XCD run; images are unchanged by both) and returns the choices."""
+    handle, structs, vstructs, index, slots, images, stream, keep = _marshal(scenes, views)
+    handle.autotune(structs, vstructs, index, stream, frames, slots)
+    del keep
+    sizes = sorted({(bv.settings.width, bv.settings.height) for bv in views})
+    return {"digit_bits": {f"{w}x{h}": handle.tuned_digit_bits(w, h) for w, h in sizes},
+            "blend_xcd_run": handle.get_option(_lib.OPT_BLEND_XCD_RUN)}

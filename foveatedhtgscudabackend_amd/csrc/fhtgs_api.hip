@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -101,6 +102,14 @@ struct PreGroup {
 
 enum FrameMode { kModeExact = 0, kModeDevice = 1, kModeGraph = 2 };
 
+struct GraphEntry {
+    std::vector<int64_t> sig;
+    hipGraphExec_t exec = nullptr;
+    uint64_t last_use = 0;
+};
+constexpr size_t kMaxGraphs = 64;
+constexpr int kMaxSlots = 4096;
+
 }  // namespace
 
 struct fhtgs_context {
@@ -123,13 +132,26 @@ struct fhtgs_context {
     DeviceBuffer params;
     void* params_host = nullptr;
     size_t params_host_cap = 0;
-    // graph replay of device-count frames
+    // the slots of the views of the call in progress (view v -> slot)
+    std::vector<ViewSlot*> cur;
+    // graph replay of device-count frames: captured graphs by launch
+    // signature, least recently used evicted beyond kMaxGraphs
     hipStream_t cap_stream = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    std::vector<int64_t> gsig;
+    std::vector<GraphEntry> graphs;
+    uint64_t graph_clock = 0;
+    // autotuned sort digit width per view size (n_small); others use digit_bits
+    std::vector<std::pair<int64_t, int>> tuned_bits;
 };
 
 namespace {
+
+// Graph capture on one handle's stream and allocation or freeing of device
+// or pinned memory by another handle must not overlap (the runtime refuses
+// such calls, or waits on the capturing stream): both hold this lock.
+std::mutex& device_memory_lock() {
+    static std::mutex m;
+    return m;
+}
 
 int set_err(fhtgs_context* h, int code, const std::string& msg) {
     if (h) h->err = msg;
@@ -145,10 +167,14 @@ int set_err(fhtgs_context* h, int code, const std::string& msg) {
 
 int ensure(fhtgs_context* h, DeviceBuffer& b, size_t bytes) {
     if (bytes <= b.cap) return FHTGS_OK;
+    std::lock_guard<std::mutex> lock(device_memory_lock());
     size_t want = std::max(bytes, b.cap + b.cap / 4);
     want = align_up(want);
     if (b.ptr) {
-        HCHK(hipDeviceSynchronize());
+        // the handle's last frame may still use the buffer; other handles'
+        // work is none of its business (a device-wide wait would also fail
+        // while another thread captures a graph)
+        if (h->have_done) HCHK(hipEventSynchronize(h->done));
         HCHK(hipFree(b.ptr));
         b.ptr = nullptr;
         b.cap = 0;
@@ -313,12 +339,14 @@ std::vector<int64_t> launch_signature(const fhtgs_context* h, const std::vector<
         g.push_back(views[pg.v[0]].anti_aliasing ? 1 : 0);
     }
     for (int v = 0; v < n_views; ++v) {
-        const ViewSlot& sl = h->slots[v];
+        const ViewSlot& sl = (*h->cur[v]);
         g.push_back(sl.vp.width);
         g.push_back(sl.vp.height);
         g.push_back(views[v].K);
         g.push_back(views[v].clear_output ? 1 : 0);
         g.push_back(sl.ib.capacity);
+        g.push_back(sl.plan.passes);
+        g.push_back(sl.plan.bits[0]);
         g.push_back((int64_t)(uintptr_t)sl.fixed.ptr);
         g.push_back((int64_t)(uintptr_t)sl.inst.ptr);
     }
@@ -331,20 +359,20 @@ int enqueue_phase1(fhtgs_context* h, const std::vector<PreGroup>& groups, const 
                    const std::vector<const SceneArgs*>& dscenes, int n_views, bool timed, hipStream_t s) {
     int rc = 0;
     if (timed && (rc = record(h, 0, s))) return rc;
-    for (int v = 0; v < n_views; ++v) HCHK(launch_clear_frame(h->slots[v].vp, h->slots[v].vb, s));
+    for (int v = 0; v < n_views; ++v) HCHK(launch_clear_frame((*h->cur[v]).vp, (*h->cur[v]).vb, s));
     if (timed && (rc = record(h, 1, s))) return rc;
-    for (int v = 0; v < n_views; ++v) HCHK(launch_tiles(h->slots[v].vp, h->slots[v].dvp, h->slots[v].vb, s));
+    for (int v = 0; v < n_views; ++v) HCHK(launch_tiles((*h->cur[v]).vp, (*h->cur[v]).dvp, (*h->cur[v]).vb, s));
     if (timed && (rc = record(h, 2, s))) return rc;
     for (const PreGroup& pg : groups) {
-        const ViewParams* hv[2] = {&h->slots[pg.v[0]].vp, &h->slots[pg.v[pg.nv - 1]].vp};
-        const ViewParams* dv[2] = {h->slots[pg.v[0]].dvp, h->slots[pg.v[pg.nv - 1]].dvp};
-        const ViewBuffers vbs[2] = {h->slots[pg.v[0]].vb, h->slots[pg.v[pg.nv - 1]].vb};
+        const ViewParams* hv[2] = {&(*h->cur[pg.v[0]]).vp, &(*h->cur[pg.v[pg.nv - 1]]).vp};
+        const ViewParams* dv[2] = {(*h->cur[pg.v[0]]).dvp, (*h->cur[pg.v[pg.nv - 1]]).dvp};
+        const ViewBuffers vbs[2] = {(*h->cur[pg.v[0]]).vb, (*h->cur[pg.v[pg.nv - 1]]).vb};
         HCHK(launch_preprocess(hv, dv, vbs, pg.nv, dscenes[pg.scene], (uint32_t)scenes[pg.scene].n_primitives, s));
     }
     if (timed && (rc = record(h, 3, s))) return rc;
     for (int v = 0; v < n_views; ++v) {
-        const int64_t N = h->slots[v].n_prims;
-        if (N > 0) HCHK(launch_scan_blocks(h->slots[v].vb, (uint32_t)((N + kPrimBlock - 1) / kPrimBlock), s));
+        const int64_t N = (*h->cur[v]).n_prims;
+        if (N > 0) HCHK(launch_scan_blocks((*h->cur[v]).vb, (uint32_t)((N + kPrimBlock - 1) / kPrimBlock), s));
     }
     return FHTGS_OK;
 }
@@ -357,7 +385,7 @@ int enqueue_phase2(fhtgs_context* h, const fhtgs_view* views, int n_views, bool 
     int rc = 0;
     if (timed && (rc = record(h, 5, s))) return rc;
     for (int v = 0; v < n_views; ++v) {
-        ViewSlot& slot = h->slots[v];
+        ViewSlot& slot = (*h->cur[v]);
         const int64_t tiles = exact ? ((int64_t)slot.n_instances + kSortTile - 1) / kSortTile : slot.ib.tiles_cap;
         HCHK(launch_sort_clear(slot.ib, slot.plan, tiles, s));
         HCHK(launch_instances(slot.dvp, slot.vb, slot.ib, slot.plan, (uint32_t)slot.n_prims, s));
@@ -369,7 +397,7 @@ int enqueue_phase2(fhtgs_context* h, const fhtgs_view* views, int n_views, bool 
     }
     if (timed && (rc = record(h, 6, s))) return rc;
     for (int v = 0; v < n_views; ++v) {
-        ViewSlot& slot = h->slots[v];
+        ViewSlot& slot = (*h->cur[v]);
         const int64_t tiles = exact ? ((int64_t)slot.n_instances + kSortTile - 1) / kSortTile : slot.ib.tiles_cap;
         int src = 0;
         for (int p = 0; p < slot.plan.passes; ++p) {
@@ -380,20 +408,20 @@ int enqueue_phase2(fhtgs_context* h, const fhtgs_view* views, int n_views, bool 
     }
     if (timed && (rc = record(h, 7, s))) return rc;
     for (int v = 0; v < n_views; ++v) {
-        ViewSlot& slot = h->slots[v];
+        ViewSlot& slot = (*h->cur[v]);
         HCHK(launch_ranges(slot.vb, slot.ib, slot.plan, slot.sorted_src,
                            exact ? slot.n_instances : (uint32_t)slot.ib.capacity, s));
     }
     if (timed && (rc = record(h, 8, s))) return rc;
     for (int v = 0; v < n_views; ++v) {
-        ViewSlot& slot = h->slots[v];
+        ViewSlot& slot = (*h->cur[v]);
         if (views[v].clear_output) HCHK(launch_clear_image(slot.vp, slot.dvp, slot.vb, s));
         HCHK(launch_blend(slot.dvp, slot.vb, slot.ib.vals[slot.sorted_src], views[v].K,
                           exact ? slot.n_active : (uint32_t)slot.vp.n_small, h->xcd_run, s));
     }
     if (timed && (rc = record(h, 9, s))) return rc;
     for (int v = 0; v < n_views; ++v) {
-        ViewSlot& slot = h->slots[v];
+        ViewSlot& slot = (*h->cur[v]);
         uint32_t grid = (uint32_t)slot.vp.n_large;
         if (exact) {
             const uint32_t* part = slot.host_fi + FI_PART;
@@ -406,9 +434,15 @@ int enqueue_phase2(fhtgs_context* h, const fhtgs_view* views, int n_views, bool 
 }
 
 // Instance buffers of a view for `cap` instances.
+int digit_bits_for(const fhtgs_context* h, int64_t n_small) {
+    for (const auto& t : h->tuned_bits)
+        if (t.first == n_small) return t.second;
+    return h->digit_bits;
+}
+
 int size_instances(fhtgs_context* h, ViewSlot& slot, int64_t cap) {
     const int end_bit = end_bit_of((uint32_t)slot.vp.n_small + 1u);
-    slot.plan = make_plan(end_bit, h->digit_bits);
+    slot.plan = make_plan(end_bit, digit_bits_for(h, slot.vp.n_small));
     size_t bytes = 0;
     carve_inst(nullptr, cap, slot.plan, slot.ib, bytes);
     int rc = ensure(h, slot.inst, bytes);
@@ -418,7 +452,8 @@ int size_instances(fhtgs_context* h, ViewSlot& slot, int64_t cap) {
 }
 
 int render_impl(fhtgs_context* h, const fhtgs_scene* scenes, int n_scenes, const fhtgs_view* views,
-                const int32_t* view_scene, int n_views, bool pair_views, hipStream_t s, int force_mode) {
+                const int32_t* view_scene, const int32_t* view_slot, int n_views, bool pair_views,
+                hipStream_t s, int force_mode) {
     // ---- preprocess groups: consecutive views of one scene that agree on
     // the preprocess parameters share one pass
     std::vector<PreGroup> groups;
@@ -432,7 +467,14 @@ int render_impl(fhtgs_context* h, const fhtgs_scene* scenes, int n_scenes, const
         groups.push_back(pg);
         v += pg.nv;
     }
-    if ((int)h->slots.size() < n_views) h->slots.resize(n_views);
+    // view v renders into slot view_slot[v] (default v): a slot keeps its
+    // buffers and instance capacity from call to call
+    int max_slot = n_views - 1;
+    if (view_slot)
+        for (int v = 0; v < n_views; ++v) max_slot = std::max(max_slot, (int)view_slot[v]);
+    if ((int)h->slots.size() <= max_slot) h->slots.resize((size_t)max_slot + 1);
+    h->cur.resize((size_t)n_views);
+    for (int v = 0; v < n_views; ++v) h->cur[(size_t)v] = &h->slots[(size_t)(view_slot ? view_slot[v] : v)];
 
     // ---- parameters: host copies, then one upload
     const size_t vp_stride = align_up(sizeof(ViewParams));
@@ -441,6 +483,7 @@ int render_impl(fhtgs_context* h, const fhtgs_scene* scenes, int n_scenes, const
     int rc = ensure(h, h->params, pbytes);
     if (rc) return rc;
     if (h->params_host_cap < pbytes) {
+        std::lock_guard<std::mutex> lock(device_memory_lock());
         if (h->params_host) HCHK(hipHostFree(h->params_host));
         h->params_host = nullptr;
         h->params_host_cap = 0;
@@ -460,7 +503,7 @@ int render_impl(fhtgs_context* h, const fhtgs_scene* scenes, int n_scenes, const
         dscenes[(size_t)k] = (const SceneArgs*)(pd + off);
     }
     for (int v = 0; v < n_views; ++v) {
-        ViewSlot& slot = h->slots[v];
+        ViewSlot& slot = (*h->cur[v]);
         const fhtgs_scene& sc = scenes[view_scene[v]];
         fill_params(sc, views[v], slot.vp);
         std::memcpy(ph + vp_stride * (size_t)v, &slot.vp, sizeof(ViewParams));
@@ -470,7 +513,10 @@ int render_impl(fhtgs_context* h, const fhtgs_scene* scenes, int n_scenes, const
         carve_fixed(nullptr, N, slot.vp.n_large, slot.vp.n_small, slot.vb, bytes);
         if ((rc = ensure(h, slot.fixed, bytes))) return rc;
         carve_fixed(slot.fixed.ptr, N, slot.vp.n_large, slot.vp.n_small, slot.vb, bytes);
-        if (!slot.host_fi) HCHK(hipHostMalloc((void**)&slot.host_fi, FI_WORDS * sizeof(uint32_t), hipHostMallocDefault));
+        if (!slot.host_fi) {
+            std::lock_guard<std::mutex> lock(device_memory_lock());
+            HCHK(hipHostMalloc((void**)&slot.host_fi, FI_WORDS * sizeof(uint32_t), hipHostMallocDefault));
+        }
         slot.n_prims = N;
     }
 
@@ -478,13 +524,13 @@ int render_impl(fhtgs_context* h, const fhtgs_scene* scenes, int n_scenes, const
     int mode = force_mode >= 0 ? force_mode : h->frame_mode;
     if (h->keep_unsorted) mode = kModeExact;
     for (int v = 0; v < n_views && mode != kModeExact; ++v) {
-        const ViewSlot& slot = h->slots[v];
+        const ViewSlot& slot = (*h->cur[v]);
         if (slot.cap_hint <= 0 || slot.cap_for_n != slot.n_prims || slot.cap_for_tiles != slot.vp.n_small)
             mode = kModeExact;
     }
     if (mode != kModeExact) {
         for (int v = 0; v < n_views; ++v)
-            if ((rc = size_instances(h, h->slots[v], h->slots[v].cap_hint))) return rc;
+            if ((rc = size_instances(h, (*h->cur[v]), (*h->cur[v]).cap_hint))) return rc;
     }
     const bool timed = h->timing != 0 && mode != kModeGraph;
     HCHK(hipMemcpyAsync(h->params.ptr, h->params_host, pbytes, hipMemcpyHostToDevice, s));
@@ -492,12 +538,12 @@ int render_impl(fhtgs_context* h, const fhtgs_scene* scenes, int n_scenes, const
     if (mode == kModeExact) {
         if ((rc = enqueue_phase1(h, groups, scenes, dscenes, n_views, timed, s))) return rc;
         for (int v = 0; v < n_views; ++v)
-            HCHK(hipMemcpyAsync(h->slots[v].host_fi, h->slots[v].vb.frame_info, FI_WORDS * sizeof(uint32_t),
+            HCHK(hipMemcpyAsync((*h->cur[v]).host_fi, (*h->cur[v]).vb.frame_info, FI_WORDS * sizeof(uint32_t),
                                 hipMemcpyDeviceToHost, s));
         if (timed && (rc = record(h, 4, s))) return rc;
         HCHK(hipStreamSynchronize(s));
         for (int v = 0; v < n_views; ++v) {
-            ViewSlot& slot = h->slots[v];
+            ViewSlot& slot = (*h->cur[v]);
             const uint32_t I = slot.n_prims > 0 ? slot.host_fi[FI_INSTANCES] : 0u;
             if (I >= (1u << 30)) return set_err(h, FHTGS_E_INVALID, "more than 2^30 tile instances in one view");
             slot.n_instances = I;
@@ -518,12 +564,17 @@ int render_impl(fhtgs_context* h, const fhtgs_scene* scenes, int n_scenes, const
     } else {
         if (h->timing) HCHK(hipEventRecord(h->ev[0], s));
         std::vector<int64_t> sig = launch_signature(h, groups, scenes, views, n_views);
-        if (!h->gexec || sig != h->gsig) {
-            if (h->gexec) {
-                HCHK(hipGraphExecDestroy(h->gexec));
-                h->gexec = nullptr;
+        GraphEntry* ge = nullptr;
+        for (GraphEntry& e : h->graphs)
+            if (e.sig == sig) ge = &e;
+        if (!ge) {
+            std::lock_guard<std::mutex> lock(device_memory_lock());
+            if (h->graphs.size() >= kMaxGraphs) {
+                auto lru = std::min_element(h->graphs.begin(), h->graphs.end(),
+                                            [](const GraphEntry& x, const GraphEntry& y) { return x.last_use < y.last_use; });
+                HCHK(hipGraphExecDestroy(lru->exec));
+                h->graphs.erase(lru);
             }
-            h->gsig.clear();
             HCHK(hipStreamBeginCapture(h->cap_stream, hipStreamCaptureModeThreadLocal));
             int crc = enqueue_phase1(h, groups, scenes, dscenes, n_views, false, h->cap_stream);
             if (!crc) crc = enqueue_phase2(h, views, n_views, false, false, h->cap_stream);
@@ -534,21 +585,22 @@ int render_impl(fhtgs_context* h, const fhtgs_scene* scenes, int n_scenes, const
                 return crc;
             }
             if (ce != hipSuccess) return set_err(h, FHTGS_E_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
-            const hipError_t ie = hipGraphInstantiate(&h->gexec, graph, nullptr, nullptr, 0);
+            GraphEntry e;
+            e.sig = sig;
+            const hipError_t ie = hipGraphInstantiate(&e.exec, graph, nullptr, nullptr, 0);
             (void)hipGraphDestroy(graph);
-            if (ie != hipSuccess) {
-                h->gexec = nullptr;
-                return set_err(h, FHTGS_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
-            }
-            h->gsig = sig;
+            if (ie != hipSuccess) return set_err(h, FHTGS_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
+            h->graphs.push_back(e);
+            ge = &h->graphs.back();
         }
-        HCHK(hipGraphLaunch(h->gexec, s));
-        for (int v = 0; v < n_views; ++v) h->slots[v].sorted_src = h->slots[v].plan.passes & 1;
+        ge->last_use = ++h->graph_clock;
+        HCHK(hipGraphLaunch(ge->exec, s));
+        for (int v = 0; v < n_views; ++v) (*h->cur[v]).sorted_src = (*h->cur[v]).plan.passes & 1;
         if (h->timing) HCHK(hipEventRecord(h->ev[FHTGS_NUM_STAGES - 1], s));
     }
     // frame info at the end: counts, valid and visible totals, error bits
     for (int v = 0; v < n_views; ++v)
-        HCHK(hipMemcpyAsync(h->slots[v].host_fi, h->slots[v].vb.frame_info, FI_WORDS * sizeof(uint32_t),
+        HCHK(hipMemcpyAsync((*h->cur[v]).host_fi, (*h->cur[v]).vb.frame_info, FI_WORDS * sizeof(uint32_t),
                             hipMemcpyDeviceToHost, s));
     HCHK(hipEventRecord(h->done, s));
     h->have_done = true;
@@ -558,7 +610,7 @@ int render_impl(fhtgs_context* h, const fhtgs_scene* scenes, int n_scenes, const
 
     bool redo = false;
     for (int v = 0; v < n_views; ++v) {
-        ViewSlot& slot = h->slots[v];
+        ViewSlot& slot = (*h->cur[v]);
         const uint32_t* fi = slot.host_fi;
         if (fi[FI_ERROR] & 1u)
             return set_err(h, FHTGS_E_TABLES,
@@ -587,7 +639,7 @@ int render_impl(fhtgs_context* h, const fhtgs_scene* scenes, int n_scenes, const
     if (redo) {
         // a view outgrew its instance capacity: render the frame again with
         // exact buffers (the next frames use the new capacity)
-        return render_impl(h, scenes, n_scenes, views, view_scene, n_views, pair_views, s, kModeExact);
+        return render_impl(h, scenes, n_scenes, views, view_scene, view_slot, n_views, pair_views, s, kModeExact);
     }
     if (h->timing) {
         // graph frames time the whole frame only
@@ -645,7 +697,8 @@ int fhtgs_destroy(fhtgs_handle h) {
     int prev = 0;
     (void)hipGetDevice(&prev);
     (void)hipSetDevice(h->device);
-    (void)hipDeviceSynchronize();
+    if (h->have_done) (void)hipEventSynchronize(h->done);
+    std::lock_guard<std::mutex> lock(device_memory_lock());
     for (auto& s : h->slots) {
         if (s.fixed.ptr) (void)hipFree(s.fixed.ptr);
         if (s.inst.ptr) (void)hipFree(s.inst.ptr);
@@ -655,7 +708,7 @@ int fhtgs_destroy(fhtgs_handle h) {
     if (h->sort_scratch.ptr) (void)hipFree(h->sort_scratch.ptr);
     if (h->params.ptr) (void)hipFree(h->params.ptr);
     if (h->params_host) (void)hipHostFree(h->params_host);
-    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+    for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.exec);
     if (h->cap_stream) (void)hipStreamDestroy(h->cap_stream);
     if (h->done) (void)hipEventDestroy(h->done);
     for (int i = 0; i <= FHTGS_NUM_STAGES; ++i)
@@ -724,12 +777,13 @@ int check_scene(fhtgs_context* h, const fhtgs_scene& sc, int idx) {
     return FHTGS_OK;
 }
 
-int render_entry(fhtgs_context* h, const fhtgs_scene* scenes, int n_scenes, const fhtgs_view* views,
-                 const int32_t* view_scene, int n_views, bool pair_views, hipStream_t s) {
+int validate_call(fhtgs_context* h, const fhtgs_scene* scenes, int n_scenes, const fhtgs_view* views,
+                  const int32_t* view_scene, const int32_t* view_slot, int n_views) {
     for (int k = 0; k < n_scenes; ++k) {
         const int rc = check_scene(h, scenes[k], k);
         if (rc) return rc;
     }
+    std::vector<char> used;
     for (int v = 0; v < n_views; ++v) {
         const int rc = validate_view(h, views[v], v);
         if (rc) return rc;
@@ -737,16 +791,38 @@ int render_entry(fhtgs_context* h, const fhtgs_scene* scenes, int n_scenes, cons
             return set_err(h, FHTGS_E_INVALID, "view " + std::to_string(v) + ": scene index out of range");
         if (views[v].active_sh_bases > 1 && views[v].active_sh_bases > scenes[view_scene[v]].total_sh_bases + 1)
             return set_err(h, FHTGS_E_INVALID, "active_sh_bases exceeds the SH coefficients given");
+        if (view_slot) {
+            const int sl = view_slot[v];
+            if (sl < 0 || sl >= kMaxSlots)
+                return set_err(h, FHTGS_E_INVALID, "view " + std::to_string(v) + ": slot out of range");
+            if ((int)used.size() <= sl) used.resize((size_t)sl + 1, 0);
+            if (used[(size_t)sl]) return set_err(h, FHTGS_E_INVALID, "two views of one call share a slot");
+            used[(size_t)sl] = 1;
+        }
     }
-    int prev_dev = 0;
-    HCHK(hipGetDevice(&prev_dev));
-    if (prev_dev != h->device) HCHK(hipSetDevice(h->device));
-    struct Restore {
-        int d, cur;
-        ~Restore() { if (d != cur) (void)hipSetDevice(d); }
-    } restore{prev_dev, h->device};
+    return FHTGS_OK;
+}
+
+struct DeviceScope {
+    int prev = 0, dev = 0;
+    bool ok = false;
+    DeviceScope(int d) : dev(d) {
+        ok = hipGetDevice(&prev) == hipSuccess && (prev == d || hipSetDevice(d) == hipSuccess);
+    }
+    ~DeviceScope() {
+        if (ok && prev != dev) (void)hipSetDevice(prev);
+    }
+};
+
+int render_entry(fhtgs_context* h, const fhtgs_scene* scenes, int n_scenes, const fhtgs_view* views,
+                 const int32_t* view_scene, const int32_t* view_slot, int n_views, bool pair_views,
+                 hipStream_t s, int force_mode) {
+    int rc = validate_call(h, scenes, n_scenes, views, view_scene, view_slot, n_views);
+    if (rc) return rc;
+    DeviceScope scope(h->device);
+    if (!scope.ok) return set_err(h, FHTGS_E_HIP, "cannot select the handle's device");
     if (h->have_done) HCHK(hipStreamWaitEvent(s, h->done, 0));
-    return render_impl(h, scenes, n_scenes, views, view_scene, n_views, pair_views, s, -1);
+    return render_impl(h, scenes, n_scenes, views, view_scene, view_slot, n_views, pair_views, s, force_mode);
 }
 
 }  // namespace
@@ -760,16 +836,120 @@ int fhtgs_render(fhtgs_handle h, const fhtgs_scene* scene, const fhtgs_view* vie
     if (share_preprocess && n_views == 2 && !same_preprocess(views[0], views[1]))
         return set_err(h, FHTGS_E_INVALID, "shared-preprocess views disagree on preprocess parameters");
     std::vector<int32_t> vs((size_t)n_views, 0);
-    return render_entry(h, scene, 1, views, vs.data(), n_views, share_preprocess != 0,
-                        (hipStream_t)stream_ptr);
+    return render_entry(h, scene, 1, views, vs.data(), nullptr, n_views, share_preprocess != 0,
+                        (hipStream_t)stream_ptr, -1);
 }
 
 int fhtgs_render_batch(fhtgs_handle h, const fhtgs_scene* scenes, int32_t n_scenes, const fhtgs_view* views,
-                       const int32_t* view_scene, int32_t n_views, void* stream_ptr) {
+                       const int32_t* view_scene, const int32_t* view_slot, int32_t n_views, void* stream_ptr) {
     if (!h) return FHTGS_E_INVALID;
     if (!scenes || n_scenes <= 0 || !views || !view_scene || n_views <= 0)
         return set_err(h, FHTGS_E_INVALID, "null scenes/views/view_scene or empty batch");
-    return render_entry(h, scenes, n_scenes, views, view_scene, n_views, true, (hipStream_t)stream_ptr);
+    return render_entry(h, scenes, n_scenes, views, view_scene, view_slot, n_views, true,
+                        (hipStream_t)stream_ptr, -1);
+}
+
+int fhtgs_autotune(fhtgs_handle h, const fhtgs_scene* scenes, int32_t n_scenes, const fhtgs_view* views,
+                   const int32_t* view_scene, const int32_t* view_slot, int32_t n_views, int32_t frames,
+                   void* stream_ptr) {
+    if (!h) return FHTGS_E_INVALID;
+    if (!scenes || n_scenes <= 0 || !views || !view_scene || n_views <= 0 || frames <= 0)
+        return set_err(h, FHTGS_E_INVALID, "null scenes/views/view_scene, empty batch or no frames");
+    hipStream_t s = (hipStream_t)stream_ptr;
+    // exact frame first: sizes every slot's buffers and instance capacity
+    int rc = render_entry(h, scenes, n_scenes, views, view_scene, view_slot, n_views, true, s, kModeExact);
+    if (rc) return rc;
+    DeviceScope scope(h->device);
+    if (!scope.ok) return set_err(h, FHTGS_E_HIP, "cannot select the handle's device");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HCHK(hipEventCreate(&e0));
+    HCHK(hipEventCreate(&e1));
+    auto timed = [&](const std::vector<int32_t>& sel, float& ms) -> int {
+        // device-count frames of the selected views, best of `frames`
+        std::vector<fhtgs_view> vv;
+        std::vector<int32_t> vsc, vsl;
+        for (int v : sel) {
+            vv.push_back(views[v]);
+            vsc.push_back(view_scene[v]);
+            vsl.push_back(view_slot ? view_slot[v] : v);
+        }
+        ms = 1e30f;
+        for (int f = 0; f <= frames; ++f) {
+            HCHK(hipEventRecord(e0, s));
+            int r = render_impl(h, scenes, n_scenes, vv.data(), vsc.data(), vsl.data(), (int)vv.size(), true, s,
+                                kModeDevice);
+            if (r) return r;
+            HCHK(hipEventRecord(e1, s));
+            HCHK(hipEventSynchronize(e1));
+            float t = 0.0f;
+            HCHK(hipEventElapsedTime(&t, e0, e1));
+            if (f > 0) ms = std::min(ms, t);  // the first run is a warm-up
+        }
+        return FHTGS_OK;
+    };
+    // sort digit width per view size: 8-bit digits (more, cheaper passes) or
+    // 11-bit digits (fewest passes); images do not depend on it
+    std::vector<int64_t> sizes;
+    for (int v = 0; v < n_views; ++v) {
+        const int64_t ns = (int64_t)((views[v].width + kTileL - 1) / kTileL) * ((views[v].height + kTileL - 1) / kTileL) * kNumSub;
+        if (std::find(sizes.begin(), sizes.end(), ns) == sizes.end()) sizes.push_back(ns);
+    }
+    for (int64_t ns : sizes) {
+        std::vector<int32_t> sel;
+        for (int v = 0; v < n_views; ++v) {
+            const int64_t vs = (int64_t)((views[v].width + kTileL - 1) / kTileL) * ((views[v].height + kTileL - 1) / kTileL) * kNumSub;
+            if (vs == ns) sel.push_back(v);
+        }
+        int best_bits = 11;
+        float best = 1e30f;
+        for (int bits : {8, 11}) {
+            auto it = std::find_if(h->tuned_bits.begin(), h->tuned_bits.end(), [&](const std::pair<int64_t, int>& t) { return t.first == ns; });
+            if (it == h->tuned_bits.end()) h->tuned_bits.push_back({ns, bits});
+            else it->second = bits;
+            float ms = 0.0f;
+            if ((rc = timed(sel, ms))) break;
+            if (ms < best) { best = ms; best_bits = bits; }
+        }
+        if (rc) break;
+        for (auto& t : h->tuned_bits)
+            if (t.first == ns) t.second = best_bits;
+    }
+    // blend tile runs per XCD, over the whole batch
+    if (!rc) {
+        std::vector<int32_t> all;
+        for (int v = 0; v < n_views; ++v) all.push_back(v);
+        int best_run = h->xcd_run;
+        float best = 1e30f;
+        for (int run : {0, 16, 64}) {
+            h->xcd_run = run;
+            float ms = 0.0f;
+            if ((rc = timed(all, ms))) break;
+            if (ms < best) { best = ms; best_run = run; }
+        }
+        h->xcd_run = best_run;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+int fhtgs_get_option(fhtgs_handle h, int32_t option, int64_t* value) {
+    if (!h || !value) return FHTGS_E_INVALID;
+    switch (option) {
+        case FHTGS_OPT_KEEP_UNSORTED: *value = h->keep_unsorted; return FHTGS_OK;
+        case FHTGS_OPT_SORT_DIGIT_BITS: *value = h->digit_bits; return FHTGS_OK;
+        case FHTGS_OPT_STAGE_TIMING: *value = h->timing; return FHTGS_OK;
+        case FHTGS_OPT_BLEND_XCD_RUN: *value = h->xcd_run; return FHTGS_OK;
+        case FHTGS_OPT_FRAME_MODE: *value = h->frame_mode; return FHTGS_OK;
+        default: return set_err(h, FHTGS_E_INVALID, "unknown option");
+    }
+}
+
+int fhtgs_tuned_digit_bits(fhtgs_handle h, int32_t width, int32_t height, int32_t* bits) {
+    if (!h || !bits || width <= 0 || height <= 0) return FHTGS_E_INVALID;
+    const int64_t ns = (int64_t)((width + kTileL - 1) / kTileL) * ((height + kTileL - 1) / kTileL) * kNumSub;
+    *bits = digit_bits_for(h, ns);
+    return FHTGS_OK;
 }
 
 int fhtgs_debug_counts(fhtgs_handle h, int32_t view, int64_t* counts) {
@@ -813,7 +993,7 @@ int fhtgs_debug_read(fhtgs_handle h, int32_t view, int32_t id, void* dst, int64_
     int prev = 0;
     HCHK(hipGetDevice(&prev));
     HCHK(hipSetDevice(h->device));
-    HCHK(hipDeviceSynchronize());
+    if (h->have_done) HCHK(hipEventSynchronize(h->done));
     if (bytes > 0) {
         if (widen) {
             std::vector<uint16_t> tmp((size_t)I);

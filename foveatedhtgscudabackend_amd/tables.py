@@ -77,6 +77,9 @@ def _u32_tensor(a: np.ndarray, device) -> torch.Tensor:
     return t.to(device)
 
 
+_ALL_VISIBLE: dict = {}
+
+
 def build_tables(width: int, height: int, device="cuda", visible: np.ndarray | None = None,
                  foveation_radius_px: int = _lib.DEFAULT_FOVEATION_RADIUS_PX,
                  blend_width_px: int = _lib.DEFAULT_BLEND_WIDTH_PX, fovea_radius_tiles: int | None = None):
@@ -85,6 +88,15 @@ def build_tables(width: int, height: int, device="cuda", visible: np.ndarray | N
     uses for ``foveation_radius_px`` (``-1`` = foveation off)."""
     if fovea_radius_tiles is None:
         fovea_radius_tiles = fovea_radius_tiles_for(width, height, foveation_radius_px, blend_width_px)
+    if visible is None:
+        # all-visible tables depend only on the size and radius: built once per device
+        key = (int(width), int(height), str(device), int(fovea_radius_tiles))
+        hit = _ALL_VISIBLE.get(key)
+        if hit is None:
+            hit = build_tables(width, height, device, np.ones(grid_size(width, height)[::-1], dtype=bool),
+                               fovea_radius_tiles=fovea_radius_tiles)
+            _ALL_VISIBLE[key] = hit
+        return hit
     words = visibility_mask_np(width, height, visible)
     sat = visibility_area_table_np(words, width, height)
     fsat = fovea_area_table_np(fovea_radius_tiles)

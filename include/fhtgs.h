@@ -22,6 +22,7 @@
  *   fhtgs_render_batch        <- no reference counterpart: many views of several
  *                                resident scenes per call (the reference renders one
  *                                view or one stereo pair per call)
+ *   fhtgs_autotune, fhtgs_tuned_digit_bits, fhtgs_get_option: no reference counterpart
  *   fhtgs_workspace_bytes     <- the per-call scratch sizing of required<>/from_blob,
  *                                rasterization/include/utils/buffer_utils.h:10-143
  *   fhtgs_sort_pairs          <- cub::DeviceRadixSort::SortPairs over bits [0,end_bit),
@@ -127,12 +128,30 @@ int fhtgs_render(fhtgs_handle h, const fhtgs_scene* scene, const fhtgs_view* vie
                  int32_t n_views, int32_t share_preprocess, void* stream);
 
 /* Renders n_views views of n_scenes scenes in one call: view v shows
- * scenes[view_scene[v]]. Consecutive views of one scene that agree on the
- * preprocess parameters share one preprocess pass (stereo pairs); each
- * view's image equals what fhtgs_render gives for it alone. */
+ * scenes[view_scene[v]] and uses the handle's slot view_slot[v] (NULL: slot
+ * v) -- a slot keeps its buffers and instance capacity between calls, so a
+ * viewer that always uses the same slots keeps its sort buffers resident.
+ * Slots of one call must differ (0 <= slot < 4096). Consecutive views of one
+ * scene that agree on the preprocess parameters share one preprocess pass
+ * (stereo pairs). Each view's image equals what fhtgs_render gives for it
+ * alone. fhtgs_debug_read/counts address slots. */
 int fhtgs_render_batch(fhtgs_handle h, const fhtgs_scene* scenes, int32_t n_scenes,
-                       const fhtgs_view* views, const int32_t* view_scene, int32_t n_views,
-                       void* stream);
+                       const fhtgs_view* views, const int32_t* view_scene,
+                       const int32_t* view_slot, int32_t n_views, void* stream);
+
+/* Autotuner for a batch (no reference counterpart): renders it once exactly,
+ * then times device-count frames (best of `frames`) to pick, per view size,
+ * the sort's digit width (8 or 11 bits) and, for the batch, the blend's XCD
+ * run (0, 16 or 64); the choices stay in the handle. Only parameters that
+ * leave every image unchanged are tuned: tile size and K are part of the
+ * reference's rendering definition. Leaves the batch's images rendered. */
+int fhtgs_autotune(fhtgs_handle h, const fhtgs_scene* scenes, int32_t n_scenes,
+                   const fhtgs_view* views, const int32_t* view_scene, const int32_t* view_slot,
+                   int32_t n_views, int32_t frames, void* stream);
+
+/* The digit width the handle's sort uses for an image size (tuned or the
+ * FHTGS_OPT_SORT_DIGIT_BITS default). */
+int fhtgs_tuned_digit_bits(fhtgs_handle h, int32_t width, int32_t height, int32_t* bits);
 
 /* Device scratch the handle currently holds (it grows on demand). */
 int fhtgs_workspace_bytes(fhtgs_handle h, int64_t* bytes);
@@ -193,6 +212,7 @@ int fhtgs_debug_counts(fhtgs_handle h, int32_t view, int64_t* counts);
  * rendered again in mode 0; the first frame of a new shape is always exact. */
 #define FHTGS_OPT_FRAME_MODE 5
 int fhtgs_set_option(fhtgs_handle h, int32_t option, int64_t value);
+int fhtgs_get_option(fhtgs_handle h, int32_t option, int64_t* value);
 
 /* ms per stage of the last timed frame, all views of the call together:
  * clear, tiles, preprocess, scan (+ count readback), sync (host wait for the

@@ -108,3 +108,32 @@ def test_batch_over_scenes_equals_single_renders(restore_mode):
             torch.cuda.synchronize()
             for k, (img, r) in enumerate(zip(out, ref)):
                 _same(img.cpu().numpy(), r, f"mode {mode} rep {rep} view {k}")
+
+
+def test_slots_and_autotune_leave_images_unchanged(restore_mode):
+    from foveatedhtgscudabackend_amd import scene as S
+    from foveatedhtgscudabackend_amd.batch import BatchView, autotune, render_batch
+    lib, h = restore_mode
+    scenes = [S.scene_to_torch(S.make_scene(n, seed=75 + k, cube=10.0), "cuda") for k, n in enumerate((25000, 15000))]
+    cams = [S.orbit_camera(w, hh, 80.0, seed=1080 + k) for k, (w, hh) in enumerate(((480, 320), (320, 240), (480, 320)))]
+    views = [BatchView(k % 2, S.settings_for(c, S.seeded_gaze(c, 2080 + k)), slot=40 + 3 * k) for k, c in enumerate(cams)]
+    h.set_option(lib.OPT_FRAME_MODE, lib.FRAME_EXACT)
+    ref = [_render(scenes[bv.scene], bv.settings) for bv in views]
+    h.set_option(lib.OPT_FRAME_MODE, lib.FRAME_GRAPH)
+    bits0 = h.get_option(lib.OPT_SORT_DIGIT_BITS)
+    try:
+        choice = autotune(scenes, views, frames=2)
+        print(choice)
+        assert set(choice["digit_bits"].values()) <= {8, 11}
+        assert choice["blend_xcd_run"] in (0, 16, 64)
+        for rep in range(3):
+            out = render_batch(scenes, views)
+            torch.cuda.synchronize()
+            for k, (img, r) in enumerate(zip(out, ref)):
+                _same(img.cpu().numpy(), r, f"rep {rep} view {k}")
+        # the slots hold what their views rendered
+        c = h.counts(43)
+        assert c[1] == (320 // 16) * (240 // 16)
+    finally:
+        h.set_option(lib.OPT_SORT_DIGIT_BITS, bits0)
+        h.set_option(lib.OPT_BLEND_XCD_RUN, 16)
