@@ -5,13 +5,17 @@ A single-threaded scalar C++ restatement of the reference inference path
 tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg may use it,
 and only as the checker / reported baseline -- never as the product path.
 
-Parity pins (tests/test_oracle_*.py): transcendentals against libm, tile
-classification and culling tables by brute force, projected extents against
-the closed form for an isotropic Gaussian, the blend against its closed forms
-(single Gaussian, pure OIT tail, over-compositing when K covers all
-fragments), the sort against a stable sort, upsampling on constant and
-linear images. Not pinned by any reference output: the reference ships no
-tests, fixtures or golden images and cannot be built here (CUDA only).
+Parity pins (tests/test_oracle_*.py): transcendentals against libm, the SH
+basis against the textbook real spherical harmonics, tile classification and
+culling tables by brute force, projected extents against the closed form for
+an isotropic Gaussian (and, anti-aliased, the dilated opacity and the tangent
+cone), the anti-aliased culls, instance enumeration by brute force, the sort
+against a stable sort, the blend against its closed forms (single Gaussian,
+pure OIT tail, over-compositing when K covers all fragments, order
+invariance), both background models against their closed forms, upsampling
+and blur on constant images. Not pinned by any reference output: the
+reference ships no tests, fixtures or golden images and cannot be built here
+(CUDA only).
 """
 from __future__ import annotations
 
