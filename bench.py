@@ -30,6 +30,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+# HBM bytes per blend dispatch (one eye) of the default workload, from the two
+# calibrated rocprofv3 PMC runs in profiles/stereo_3M_2064x2208_pmc_traffic.md
+# (FETCH_SIZE 285.2 MB + WRITE_SIZE 31.1 MB); reported only for that workload.
+BLEND_TRAFFIC_BYTES_DEFAULT = 316.3e6
 
 
 def percentile(xs, q):
@@ -206,9 +210,13 @@ def main() -> int:
         # the cutoff test; each valid instance is evaluated by 64 lanes
         evals = sum(64 * max(c[5], 0) for c in counts_eye)
         achieved = 28.0 * evals / (stage_ms["blend"] * 1e-3) / 1e12
+        default_workload = (args.gaussians, args.width, args.height, args.K, args.fov) == (3_000_000, 2064, 2208, 8, 100.0)
         roof = {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None, "kernel": "blend_kernel",
-                "note": "VALU fp32 (no MFMA); peak is MI355X fp32 vector/matrix 157.3 TFLOP/s"}
+                "frac": achieved / FP32_PEAK_TFLOPS,
+                "traffic": BLEND_TRAFFIC_BYTES_DEFAULT if default_workload else None, "kernel": "blend_kernel",
+                "note": "VALU-issue bound, FP32 without packed instructions (no MFMA); achieved = 28 FP32 ops x "
+                        "64 x n_valid evaluations / blend time; traffic = HBM bytes per dispatch (one eye) "
+                        "from the calibrated PMC runs"}
     else:
         ach = stage_gbs[dom] if stage_gbs.get(dom) else 0.0
         roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
