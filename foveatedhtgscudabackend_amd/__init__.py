@@ -5,7 +5,8 @@ Drop-in for the inference path of FoveatedHTGSCudaBackend: the same
 ``RasterizerSettings`` fields, backed by hand-written HIP kernels in
 ``libfhtgs.so`` (C ABI: include/fhtgs.h).
 """
-from .torch_bindings.rasterization import (BackgroundModel, FoveatedHTGSRasterizer,
+from .torch_bindings.rasterization import (BackgroundModel, FoveatedHTGSRasterizer, FoveatedHTGSVisualizer,
                                            GazeVisualizationType, RasterizerSettings)
 
-__all__ = ["FoveatedHTGSRasterizer", "RasterizerSettings", "BackgroundModel", "GazeVisualizationType"]
+__all__ = ["FoveatedHTGSRasterizer", "FoveatedHTGSVisualizer", "RasterizerSettings", "BackgroundModel",
+           "GazeVisualizationType"]

@@ -45,6 +45,7 @@ EXPORTED_SYMBOLS = (
     "fhtgs_workspace_bytes", "fhtgs_sort_pairs", "fhtgs_fovea_radius_tiles",
     "fhtgs_debug_read", "fhtgs_debug_counts", "fhtgs_set_option", "fhtgs_stage_times",
     "fhtgs_render_batch", "fhtgs_autotune", "fhtgs_tuned_digit_bits", "fhtgs_get_option",
+    "fhtgs_visualize_gaze", "fhtgs_visualize_tile_boundaries",
 )
 
 
@@ -116,6 +117,11 @@ def load() -> ctypes.CDLL:
         lib.fhtgs_tuned_digit_bits.restype = ctypes.c_int
         lib.fhtgs_get_option.argtypes = [vp, i32, ctypes.POINTER(i64)]
         lib.fhtgs_get_option.restype = ctypes.c_int
+        f32 = ctypes.c_float
+        lib.fhtgs_visualize_gaze.argtypes = [vp, vp, i32, i32, f32, f32, ctypes.POINTER(f32), i32, i32, i32, vp]
+        lib.fhtgs_visualize_gaze.restype = ctypes.c_int
+        lib.fhtgs_visualize_tile_boundaries.argtypes = [vp, vp, i32, i32, vp, f32, f32, i32, i32, i32, vp]
+        lib.fhtgs_visualize_tile_boundaries.restype = ctypes.c_int
         lib.fhtgs_workspace_bytes.argtypes = [vp, ctypes.POINTER(i64)]
         lib.fhtgs_workspace_bytes.restype = ctypes.c_int
         lib.fhtgs_sort_pairs.argtypes = [vp, vp, vp, i64, i32, i32, vp]
@@ -199,6 +205,23 @@ class Handle:
             rc = self.lib.fhtgs_autotune(self.ptr, sarr, len(scenes), varr, iarr, slots, len(views),
                                          int(frames), ctypes.c_void_p(stream))
             self.check(rc, "fhtgs_autotune")
+
+    def visualize_gaze(self, image_ptr: int, width: int, height: int, gaze: tuple, rgb: tuple, vis_type: int,
+                       size: int, to_chw: bool, stream: int) -> None:
+        col = (ctypes.c_float * 3)(*[float(c) for c in rgb])
+        with self.lock:
+            self.check(self.lib.fhtgs_visualize_gaze(self.ptr, ctypes.c_void_p(image_ptr), int(width), int(height),
+                                                     float(gaze[0]), float(gaze[1]), col, int(vis_type), int(size),
+                                                     1 if to_chw else 0, ctypes.c_void_p(stream)),
+                       "fhtgs_visualize_gaze")
+
+    def visualize_tile_boundaries(self, image_ptr: int, width: int, height: int, mask_ptr: int, gaze: tuple,
+                                  foveation_radius_px: int, blend_width_px: int, to_chw: bool, stream: int) -> None:
+        with self.lock:
+            self.check(self.lib.fhtgs_visualize_tile_boundaries(
+                self.ptr, ctypes.c_void_p(image_ptr), int(width), int(height), ctypes.c_void_p(mask_ptr),
+                float(gaze[0]), float(gaze[1]), int(foveation_radius_px), int(blend_width_px),
+                1 if to_chw else 0, ctypes.c_void_p(stream)), "fhtgs_visualize_tile_boundaries")
 
     def get_option(self, option: int) -> int:
         v = ctypes.c_int64()

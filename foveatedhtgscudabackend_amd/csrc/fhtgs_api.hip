@@ -141,6 +141,10 @@ struct fhtgs_context {
     uint64_t graph_clock = 0;
     // autotuned sort digit width per view size (n_small); others use digit_bits
     std::vector<std::pair<int64_t, int>> tuned_bits;
+    // tile-boundary overlay: its own parameters and tile buffers
+    ViewSlot vis;
+    DeviceBuffer vis_params;
+    ViewParams* vis_params_host = nullptr;
 };
 
 namespace {
@@ -706,6 +710,9 @@ int fhtgs_destroy(fhtgs_handle h) {
         if (s.host_fi) (void)hipHostFree(s.host_fi);
     }
     if (h->sort_scratch.ptr) (void)hipFree(h->sort_scratch.ptr);
+    if (h->vis.fixed.ptr) (void)hipFree(h->vis.fixed.ptr);
+    if (h->vis_params.ptr) (void)hipFree(h->vis_params.ptr);
+    if (h->vis_params_host) (void)hipHostFree(h->vis_params_host);
     if (h->params.ptr) (void)hipFree(h->params.ptr);
     if (h->params_host) (void)hipHostFree(h->params_host);
     for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.exec);
@@ -949,6 +956,75 @@ int fhtgs_tuned_digit_bits(fhtgs_handle h, int32_t width, int32_t height, int32_
     if (!h || !bits || width <= 0 || height <= 0) return FHTGS_E_INVALID;
     const int64_t ns = (int64_t)((width + kTileL - 1) / kTileL) * ((height + kTileL - 1) / kTileL) * kNumSub;
     *bits = digit_bits_for(h, ns);
+    return FHTGS_OK;
+}
+
+int fhtgs_visualize_gaze(fhtgs_handle h, float* image, int32_t width, int32_t height, float gaze_x,
+                         float gaze_y, const float* rgb, int32_t type, int32_t size, int32_t to_chw,
+                         void* stream_ptr) {
+    if (!h) return FHTGS_E_INVALID;
+    if (!image || !rgb || width <= 0 || height <= 0) return set_err(h, FHTGS_E_INVALID, "null image/colour or bad size");
+    if (type < 0 || type > 2) return set_err(h, FHTGS_E_INVALID, "visualization type must be 0, 1 or 2");
+    DeviceScope scope(h->device);
+    if (!scope.ok) return set_err(h, FHTGS_E_HIP, "cannot select the handle's device");
+    hipStream_t s = (hipStream_t)stream_ptr;
+    if (h->have_done) HCHK(hipStreamWaitEvent(s, h->done, 0));
+    HCHK(launch_gaze_marker(image, width, height, gaze_x, gaze_y, rgb, size, type, to_chw ? 1 : 0, s));
+    HCHK(hipEventRecord(h->done, s));
+    h->have_done = true;
+    return FHTGS_OK;
+}
+
+int fhtgs_visualize_tile_boundaries(fhtgs_handle h, float* image, int32_t width, int32_t height,
+                                    const uint32_t* visibility_mask, float gaze_x, float gaze_y,
+                                    int32_t foveation_radius_px, int32_t blend_width_px, int32_t to_chw,
+                                    void* stream_ptr) {
+    if (!h) return FHTGS_E_INVALID;
+    if (!image || !visibility_mask || width <= 0 || height <= 0)
+        return set_err(h, FHTGS_E_INVALID, "null image/mask or bad size");
+    if (foveation_radius_px >= 0 && blend_width_px > foveation_radius_px)
+        return set_err(h, FHTGS_E_INVALID, "blend width exceeds the foveation radius");
+    DeviceScope scope(h->device);
+    if (!scope.ok) return set_err(h, FHTGS_E_HIP, "cannot select the handle's device");
+    hipStream_t s = (hipStream_t)stream_ptr;
+    if (h->have_done) HCHK(hipEventSynchronize(h->done));  // staging and tile buffers are reused
+    fhtgs_view v;
+    std::memset(&v, 0, sizeof(v));
+    v.width = width;
+    v.height = height;
+    v.gaze_x = gaze_x;
+    v.gaze_y = gaze_y;
+    v.visibility_mask = visibility_mask;
+    v.foveation_radius_px = foveation_radius_px;
+    v.blend_width_px = blend_width_px;
+    v.image = image;
+    fhtgs_scene sc;
+    std::memset(&sc, 0, sizeof(sc));
+    ViewSlot& sl = h->vis;
+    fill_params(sc, v, sl.vp);
+    if (sl.vp.n_large > 65536) return set_err(h, FHTGS_E_INVALID, "image too large for the tile overlay");
+    size_t bytes = 0;
+    carve_fixed(nullptr, 0, sl.vp.n_large, sl.vp.n_small, sl.vb, bytes);
+    int rc = ensure(h, sl.fixed, bytes);
+    if (rc) return rc;
+    carve_fixed(sl.fixed.ptr, 0, sl.vp.n_large, sl.vp.n_small, sl.vb, bytes);
+    if ((rc = ensure(h, h->vis_params, sizeof(ViewParams)))) return rc;
+    if (!h->vis_params_host) {
+        std::lock_guard<std::mutex> lock(device_memory_lock());
+        HCHK(hipHostMalloc((void**)&h->vis_params_host, sizeof(ViewParams), hipHostMallocDefault));
+    }
+    *h->vis_params_host = sl.vp;
+    const ViewParams* dvp = (const ViewParams*)h->vis_params.ptr;
+    if (h->have_done) HCHK(hipStreamWaitEvent(s, h->done, 0));
+    HCHK(hipMemcpyAsync(h->vis_params.ptr, h->vis_params_host, sizeof(ViewParams), hipMemcpyHostToDevice, s));
+    // the rasterizer's own tile classification (visualization.cu:60-178)
+    HCHK(launch_clear_frame(sl.vp, sl.vb, s));
+    HCHK(launch_tiles(sl.vp, dvp, sl.vb, s));
+    HCHK(launch_tile_borders(sl.vp, dvp, sl.vb, image, to_chw ? 1 : 0, s));
+    HCHK(hipEventRecord(h->done, s));
+    h->have_done = true;
+    // the reference call returns after its streams finish (visualization.cu:232-234)
+    HCHK(hipStreamSynchronize(s));
     return FHTGS_OK;
 }
 

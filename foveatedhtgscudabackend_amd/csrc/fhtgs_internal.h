@@ -145,5 +145,9 @@ hipError_t launch_clear_image(const ViewParams& vp, const ViewParams* dvp, const
                               hipStream_t s);
 hipError_t launch_widen_keys(const void* keys, int key_bytes, uint32_t* out, uint32_t n,
                              hipStream_t s);
+hipError_t launch_gaze_marker(float* img, int W, int H, float gaze_x, float gaze_y, const float rgb[3], int size,
+                              int type, int chw, hipStream_t s);
+hipError_t launch_tile_borders(const ViewParams& vp, const ViewParams* dvp, const ViewBuffers& vb, float* img,
+                               int chw, hipStream_t s);
 
 }  // namespace fhtgs

@@ -11,7 +11,7 @@ from typing import NamedTuple
 
 import torch
 
-from .. import _C, _C_benchmarking, _C_stereo
+from .. import _C, _C_benchmarking, _C_stereo, _C_visualization
 from .._lib import DEFAULT_BLEND_WIDTH_PX, DEFAULT_FOVEATION_RADIUS_PX
 
 
@@ -141,3 +141,38 @@ class FoveatedHTGSRasterizer(torch.nn.Module):
         return _C_stereo.render(positions, scales, rotations, opacities, sh_0, sh_rest,
                                 *settings_left.as_tuple(), *settings_right.as_tuple(),
                                 to_chw, blur_periphery, aaa_mode, **self._fov())
+
+
+class FoveatedHTGSVisualizer(torch.nn.Module):
+    """Debug overlays (torch_bindings/rasterization.py:157-192 of the
+    reference): the gaze marker and the tile-type borders, drawn in place.
+    The foveation radii default to the rasterizer's (375 / 125 px)."""
+
+    def __init__(self, foveation_radius_px: int = DEFAULT_FOVEATION_RADIUS_PX,
+                 blend_width_px: int = DEFAULT_BLEND_WIDTH_PX):
+        super().__init__()
+        self.foveation_radius_px = int(foveation_radius_px)
+        self.blend_width_px = int(blend_width_px)
+
+    def visualize_gaze(
+            self,
+            image: torch.Tensor,
+            gaze_position: torch.Tensor,
+            gaze_color: torch.Tensor,
+            visualization_size: int,
+            visualization_type: GazeVisualizationType,
+            to_chw: bool,
+    ) -> torch.Tensor:
+        return _C_visualization.visualize_gaze(image, gaze_position, gaze_color, visualization_type.value,
+                                               visualization_size, to_chw)
+
+    def visualize_tile_boundaries(
+            self,
+            image: torch.Tensor,
+            visibility_mask: torch.Tensor,
+            gaze_position: torch.Tensor,
+            to_chw: bool,
+    ) -> torch.Tensor:
+        return _C_visualization.visualize_tile_boundaries(image, visibility_mask, gaze_position, to_chw,
+                                                          foveation_radius_px=self.foveation_radius_px,
+                                                          blend_width_px=self.blend_width_px)

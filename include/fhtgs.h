@@ -23,6 +23,13 @@
  *                                resident scenes per call (the reference renders one
  *                                view or one stereo pair per call)
  *   fhtgs_autotune, fhtgs_tuned_digit_bits, fhtgs_get_option: no reference counterpart
+ *   fhtgs_visualize_gaze      <- htgs_foveated::visualization::visualize_gaze_wrapper,
+ *                                visualization/include/visualization_api.h:8-14,
+ *                                visualization/src/visualization_api.cu:13-35
+ *                                (_C_visualization.visualize_gaze, bindings_visualization.cpp:8)
+ *   fhtgs_visualize_tile_boundaries <- visualize_tile_boundaries_wrapper,
+ *                                visualization_api.h:16-20, visualization_api.cu:37-63
+ *                                (_C_visualization.visualize_tile_boundaries, bindings_visualization.cpp:9)
  *   fhtgs_workspace_bytes     <- the per-call scratch sizing of required<>/from_blob,
  *                                rasterization/include/utils/buffer_utils.h:10-143
  *   fhtgs_sort_pairs          <- cub::DeviceRadixSort::SortPairs over bits [0,end_bit),
@@ -160,6 +167,22 @@ int fhtgs_workspace_bytes(fhtgs_handle h, int64_t* bytes);
  * key_bytes 2 or 4, values uint32. Uses the handle's scratch. */
 int fhtgs_sort_pairs(fhtgs_handle h, void* keys, uint32_t* values, int64_t n,
                      int32_t key_bytes, int32_t end_bit, void* stream);
+
+/* Draws the gaze marker into `image` (device float32, 3*H*W, layout by
+ * to_chw): the gaze clamped to the image and truncated to a pixel, a marker of
+ * `size` pixels -- type 0 circle, 1 square, 2 cross (3 px wide arms); rgb is
+ * a host pointer to 3 floats. Enqueued on `stream`. */
+int fhtgs_visualize_gaze(fhtgs_handle h, float* image, int32_t width, int32_t height,
+                         float gaze_x, float gaze_y, const float* rgb, int32_t type, int32_t size,
+                         int32_t to_chw, void* stream);
+
+/* Draws the borders of the active tiles of the rasterizer's classification for
+ * this gaze, mask and foveation radii: FOVEA sub-tiles orange, BLENDED
+ * sub-tiles light blue, PERIPHERY tiles green. Returns after the overlay is drawn. */
+int fhtgs_visualize_tile_boundaries(fhtgs_handle h, float* image, int32_t width, int32_t height,
+                                    const uint32_t* visibility_mask, float gaze_x, float gaze_y,
+                                    int32_t foveation_radius_px, int32_t blend_width_px,
+                                    int32_t to_chw, void* stream);
 
 /* Fovea and blend-ring radii in 16-px tiles for an image; the fovea table must
  * be built for the first. foveation_radius_px < 0 means foveation off. */

@@ -109,8 +109,39 @@ def load() -> ctypes.CDLL:
     lib.orc_atan2f.restype = ctypes.c_float
     lib.orc_sh_eval.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]
     lib.orc_sh_eval.restype = None
+    lib.orc_visualize_gaze.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
+                                       ctypes.c_float, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                       ctypes.c_int32]
+    lib.orc_visualize_gaze.restype = None
+    lib.orc_visualize_tile_boundaries.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p,
+                                                  ctypes.c_float, ctypes.c_float, ctypes.c_int32, ctypes.c_int32,
+                                                  ctypes.c_int32]
+    lib.orc_visualize_tile_boundaries.restype = None
     _lib = lib
     return lib
+
+
+def visualize_gaze(image: np.ndarray, gaze, rgb, vis_type: int, size: int, chw: bool) -> np.ndarray:
+    """Draws the gaze marker into a copy of a float32 image (H,W,3) or (3,H,W)."""
+    lib = load()
+    img = np.ascontiguousarray(image, dtype=np.float32).copy()
+    H, W = (img.shape[1], img.shape[2]) if chw else (img.shape[0], img.shape[1])
+    col = np.ascontiguousarray(rgb, dtype=np.float32)
+    lib.orc_visualize_gaze(_ptr(img), W, H, float(gaze[0]), float(gaze[1]), _ptr(col), int(vis_type), int(size),
+                           int(bool(chw)))
+    return img
+
+
+def visualize_tile_boundaries(image: np.ndarray, mask_words: np.ndarray, gaze, foveation_radius_px: int = 375,
+                              blend_width_px: int = 125, chw: bool = False) -> np.ndarray:
+    """Draws the tile-type borders into a copy of a float32 image."""
+    lib = load()
+    img = np.ascontiguousarray(image, dtype=np.float32).copy()
+    H, W = (img.shape[1], img.shape[2]) if chw else (img.shape[0], img.shape[1])
+    m = np.ascontiguousarray(mask_words, dtype=np.uint32)
+    lib.orc_visualize_tile_boundaries(_ptr(img), W, H, _ptr(m), float(gaze[0]), float(gaze[1]),
+                                      int(foveation_radius_px), int(blend_width_px), int(bool(chw)))
+    return img
 
 
 def _ptr(a: np.ndarray) -> int:

@@ -110,6 +110,15 @@ void orc_result_free(orc_result* r);
 void orc_build_visibility_sat(const uint32_t* mask, int32_t gw, int32_t gh, uint32_t* sat);
 void orc_build_fovea_sat(int32_t radius_tiles, uint32_t* sat);
 
+/* Debug overlays (visualization/src/visualization.cu): the gaze marker (type
+ * 0 circle, 1 square, 2 cross) and the tile-type borders of the render's
+ * classification, drawn into a float32 image (HWC, or CHW when chw). */
+void orc_visualize_gaze(float* image, int32_t width, int32_t height, float gaze_x, float gaze_y,
+                        const float* rgb, int32_t type, int32_t size, int32_t chw);
+void orc_visualize_tile_boundaries(float* image, int32_t width, int32_t height, const uint32_t* mask,
+                                   float gaze_x, float gaze_y, int32_t foveation_radius_px,
+                                   int32_t blend_width_px, int32_t chw);
+
 /* Stable LSD reference sort: keys compared on bits [0, end_bit). */
 void orc_stable_sort_pairs(uint32_t* keys, uint32_t* vals, int64_t n, int32_t end_bit);
 
