@@ -65,9 +65,18 @@ def look_at_rotation(eye: np.ndarray, target: np.ndarray) -> np.ndarray:
 
 def make_camera(width: int, height: int, fov_x_deg: float, eye: np.ndarray, R: np.ndarray,
                 near: float = 0.2, far: float = 1000.0) -> Camera:
+    """Pinhole camera with square pixels and a centred principal point from
+    the horizontal field of view; R: world -> camera rotation (rows)."""
     fx = 0.5 * width / math.tan(math.radians(fov_x_deg) / 2)
-    fy = fx
-    cx, cy = width / 2.0, height / 2.0
+    return camera_from_intrinsics(width, height, fx, fx, width / 2.0, height / 2.0, eye, R, near, far)
+
+
+def camera_from_intrinsics(width: int, height: int, fx: float, fy: float, cx: float, cy: float,
+                           eye: np.ndarray, R: np.ndarray, near: float = 0.2, far: float = 1000.0) -> Camera:
+    """Camera matrices from pinhole intrinsics, the camera centre and the
+    world -> camera rotation R (rows: camera x right, y down, z forward)."""
+    eye = np.asarray(eye, dtype=np.float64)
+    R = np.asarray(R, dtype=np.float64)
     t = -R @ eye
     M = np.eye(4)
     M[:3, :3] = R
