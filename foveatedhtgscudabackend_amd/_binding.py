@@ -82,16 +82,27 @@ def view_struct(device, M, VPM, VPR_inv, cam_position, gaze_position, visibility
         raise ValueError("gaze_position must have 2 elements")
     v.gaze_x, v.gaze_y = float(g[0]), float(g[1])
     gw, gh = (width + 15) // 16, (height + 15) // 16
-    if visibility_mask.numel() < (gw * gh + 31) // 32:
+    # extension: both summed-area tables None -> built on the device inside
+    # the frame (and a None mask means every tile is visible)
+    auto = visibility_mask_area_table is None and fovea_mask_area_table is None
+    if (visibility_mask_area_table is None) != (fovea_mask_area_table is None):
+        raise ValueError("give both summed-area tables or neither")
+    if visibility_mask is None and not auto:
+        raise ValueError("visibility_mask is required with explicit summed-area tables")
+    if visibility_mask is not None and visibility_mask.numel() < (gw * gh + 31) // 32:
         raise ValueError("visibility_mask is smaller than ceil(tiles/32) words")
-    if visibility_mask_area_table.numel() < (gw + 1) * (gh + 1):
-        raise ValueError("visibility_mask_area_table is smaller than (gw+1)*(gh+1)")
-    R, _ = _lib.fovea_radius_tiles(width, height, int(foveation_radius_px), int(blend_width_px))
-    if fovea_mask_area_table.numel() < (2 * R + 1) ** 2:
-        raise ValueError(f"fovea_mask_area_table must hold (2*{R}+1)^2 entries")
+    if not auto:
+        if visibility_mask_area_table.numel() < (gw + 1) * (gh + 1):
+            raise ValueError("visibility_mask_area_table is smaller than (gw+1)*(gh+1)")
+        R, _ = _lib.fovea_radius_tiles(width, height, int(foveation_radius_px), int(blend_width_px))
+        if fovea_mask_area_table.numel() < (2 * R + 1) ** 2:
+            raise ValueError(f"fovea_mask_area_table must hold (2*{R}+1)^2 entries")
     for field, t in (("visibility_mask", visibility_mask),
                      ("visibility_mask_area_table", visibility_mask_area_table),
                      ("fovea_mask_area_table", fovea_mask_area_table)):
+        if t is None:
+            setattr(v, field, None)
+            continue
         p, tc = _dev_ptr(t, field, _U32, device)
         keep.append(tc)
         setattr(v, field, p)

@@ -45,7 +45,7 @@ EXPORTED_SYMBOLS = (
     "fhtgs_workspace_bytes", "fhtgs_sort_pairs", "fhtgs_fovea_radius_tiles",
     "fhtgs_debug_read", "fhtgs_debug_counts", "fhtgs_set_option", "fhtgs_stage_times",
     "fhtgs_render_batch", "fhtgs_autotune", "fhtgs_tuned_digit_bits", "fhtgs_get_option",
-    "fhtgs_visualize_gaze", "fhtgs_visualize_tile_boundaries",
+    "fhtgs_visualize_gaze", "fhtgs_visualize_tile_boundaries", "fhtgs_build_tables",
 )
 
 
@@ -122,6 +122,8 @@ def load() -> ctypes.CDLL:
         lib.fhtgs_visualize_gaze.restype = ctypes.c_int
         lib.fhtgs_visualize_tile_boundaries.argtypes = [vp, vp, i32, i32, vp, f32, f32, i32, i32, i32, vp]
         lib.fhtgs_visualize_tile_boundaries.restype = ctypes.c_int
+        lib.fhtgs_build_tables.argtypes = [vp, vp, i32, i32, i32, i32, vp, vp, vp, vp]
+        lib.fhtgs_build_tables.restype = ctypes.c_int
         lib.fhtgs_workspace_bytes.argtypes = [vp, ctypes.POINTER(i64)]
         lib.fhtgs_workspace_bytes.restype = ctypes.c_int
         lib.fhtgs_sort_pairs.argtypes = [vp, vp, vp, i64, i32, i32, vp]
@@ -222,6 +224,14 @@ class Handle:
                 self.ptr, ctypes.c_void_p(image_ptr), int(width), int(height), ctypes.c_void_p(mask_ptr),
                 float(gaze[0]), float(gaze[1]), int(foveation_radius_px), int(blend_width_px),
                 1 if to_chw else 0, ctypes.c_void_p(stream)), "fhtgs_visualize_tile_boundaries")
+
+    def build_tables(self, visible_ptr: int, width: int, height: int, foveation_radius_px: int,
+                     blend_width_px: int, mask_ptr: int, vis_sat_ptr: int, fovea_sat_ptr: int, stream: int) -> None:
+        with self.lock:
+            self.check(self.lib.fhtgs_build_tables(
+                self.ptr, ctypes.c_void_p(visible_ptr or None), int(width), int(height), int(foveation_radius_px),
+                int(blend_width_px), ctypes.c_void_p(mask_ptr or None), ctypes.c_void_p(vis_sat_ptr or None),
+                ctypes.c_void_p(fovea_sat_ptr or None), ctypes.c_void_p(stream)), "fhtgs_build_tables")
 
     def get_option(self, option: int) -> int:
         v = ctypes.c_int64()

@@ -91,6 +91,8 @@ struct ViewSlot {
     // the last exact frame of the same shape (N, n_small); 0 = unknown
     int64_t cap_hint = 0;
     int64_t cap_for_n = -1, cap_for_tiles = -1;
+    bool auto_tables = false;  // culling tables built on the device each frame
+    bool own_mask = false;     // ... including the mask (all tiles visible)
 };
 
 // One preprocess launch: a scene and one or two of its views.
@@ -207,7 +209,8 @@ struct Carver {
     }
 };
 
-void carve_fixed(void* base, int64_t N, int n_large, int n_small, ViewBuffers& vb, size_t& bytes) {
+void carve_fixed(void* base, int64_t N, int n_large, int n_small, ViewBuffers& vb, size_t& bytes,
+                 int gw = 0, int gh = 0, int R = -1) {
     Carver c(base);
     const size_t nb = (size_t)((N + kPrimBlock - 1) / kPrimBlock);
     vb.frame_info = c.take<uint32_t>(FI_WORDS);
@@ -220,6 +223,12 @@ void carve_fixed(void* base, int64_t N, int n_large, int n_small, ViewBuffers& v
     vb.bounds = c.take<ushort4>((size_t)N);
     vb.prim = c.take<float4>((size_t)N * 4);
     vb.block_sums = c.take<uint32_t>(std::max<size_t>(nb, 1));
+    vb.tab_mask = vb.tab_vis_sat = vb.tab_fovea_sat = nullptr;
+    if (R >= 0) {
+        vb.tab_mask = c.take<uint32_t>((size_t)(n_large + 31) / 32);
+        vb.tab_vis_sat = c.take<uint32_t>((size_t)(gw + 1) * (gh + 1));
+        vb.tab_fovea_sat = c.take<uint32_t>((size_t)(2 * R + 1) * (2 * R + 1));
+    }
     bytes = align_up(c.off);
 }
 
@@ -244,9 +253,13 @@ int validate_view(fhtgs_context* h, const fhtgs_view& v, int idx) {
     if (v.width <= 0 || v.height <= 0) return set_err(h, FHTGS_E_INVALID, p + "width and height must be positive");
     if ((int64_t)v.width * v.height * 3 > (int64_t)1 << 31)
         return set_err(h, FHTGS_E_INVALID, p + "image too large");
-    if (!v.M || !v.VPM || !v.cam_position || !v.visibility_mask || !v.visibility_mask_area_table ||
-        !v.fovea_mask_area_table || !v.image)
-        return set_err(h, FHTGS_E_INVALID, p + "null camera, table or image pointer");
+    if (!v.M || !v.VPM || !v.cam_position || !v.image)
+        return set_err(h, FHTGS_E_INVALID, p + "null camera or image pointer");
+    // both tables given, or neither (built on the device each frame)
+    if ((v.visibility_mask_area_table == nullptr) != (v.fovea_mask_area_table == nullptr))
+        return set_err(h, FHTGS_E_INVALID, p + "give both summed-area tables or neither");
+    if (v.visibility_mask_area_table && !v.visibility_mask)
+        return set_err(h, FHTGS_E_INVALID, p + "null visibility mask");
     if (v.blur_periphery && !v.image_temp)
         return set_err(h, FHTGS_E_INVALID, p + "blur_periphery needs image_temp");
     if (v.background_model_type < 0 || v.background_model_type > 3)
@@ -348,6 +361,8 @@ std::vector<int64_t> launch_signature(const fhtgs_context* h, const std::vector<
         g.push_back(sl.vp.height);
         g.push_back(views[v].K);
         g.push_back(views[v].clear_output ? 1 : 0);
+        g.push_back(sl.auto_tables ? 1 + (sl.own_mask ? 1 : 0) : 0);
+        g.push_back(sl.auto_tables ? sl.vp.fovea_r : -1);
         g.push_back(sl.ib.capacity);
         g.push_back(sl.plan.passes);
         g.push_back(sl.plan.bits[0]);
@@ -363,7 +378,13 @@ int enqueue_phase1(fhtgs_context* h, const std::vector<PreGroup>& groups, const 
                    const std::vector<const SceneArgs*>& dscenes, int n_views, bool timed, hipStream_t s) {
     int rc = 0;
     if (timed && (rc = record(h, 0, s))) return rc;
-    for (int v = 0; v < n_views; ++v) HCHK(launch_clear_frame((*h->cur[v]).vp, (*h->cur[v]).vb, s));
+    for (int v = 0; v < n_views; ++v) {
+        ViewSlot& sl = *h->cur[v];
+        if (sl.auto_tables)
+            HCHK(launch_build_tables(nullptr, sl.own_mask, sl.vp.gw, sl.vp.gh, sl.vp.fovea_r,
+                                     const_cast<uint32_t*>(sl.vp.vis_mask), sl.vb.tab_vis_sat, sl.vb.tab_fovea_sat, s));
+        HCHK(launch_clear_frame(sl.vp, sl.vb, s));
+    }
     if (timed && (rc = record(h, 1, s))) return rc;
     for (int v = 0; v < n_views; ++v) HCHK(launch_tiles((*h->cur[v]).vp, (*h->cur[v]).dvp, (*h->cur[v]).vb, s));
     if (timed && (rc = record(h, 2, s))) return rc;
@@ -510,13 +531,23 @@ int render_impl(fhtgs_context* h, const fhtgs_scene* scenes, int n_scenes, const
         ViewSlot& slot = (*h->cur[v]);
         const fhtgs_scene& sc = scenes[view_scene[v]];
         fill_params(sc, views[v], slot.vp);
-        std::memcpy(ph + vp_stride * (size_t)v, &slot.vp, sizeof(ViewParams));
         slot.dvp = (ViewParams*)(pd + vp_stride * (size_t)v);
         const int64_t N = sc.n_primitives;
+        // a view without summed-area tables gets them built into its slot
+        slot.auto_tables = views[v].fovea_mask_area_table == nullptr;
+        const int tR = slot.auto_tables ? slot.vp.fovea_r : -1;
         size_t bytes = 0;
-        carve_fixed(nullptr, N, slot.vp.n_large, slot.vp.n_small, slot.vb, bytes);
+        carve_fixed(nullptr, N, slot.vp.n_large, slot.vp.n_small, slot.vb, bytes, slot.vp.gw, slot.vp.gh, tR);
         if ((rc = ensure(h, slot.fixed, bytes))) return rc;
-        carve_fixed(slot.fixed.ptr, N, slot.vp.n_large, slot.vp.n_small, slot.vb, bytes);
+        carve_fixed(slot.fixed.ptr, N, slot.vp.n_large, slot.vp.n_small, slot.vb, bytes, slot.vp.gw, slot.vp.gh, tR);
+        if (slot.auto_tables) {
+            if (slot.vp.n_large > 65536) return set_err(h, FHTGS_E_INVALID, "image too large for device-built tables");
+            slot.own_mask = views[v].visibility_mask == nullptr;
+            if (slot.own_mask) slot.vp.vis_mask = slot.vb.tab_mask;
+            slot.vp.vis_sat = slot.vb.tab_vis_sat;
+            slot.vp.fovea_sat = slot.vb.tab_fovea_sat;
+        }
+        std::memcpy(ph + vp_stride * (size_t)v, &slot.vp, sizeof(ViewParams));
         if (!slot.host_fi) {
             std::lock_guard<std::mutex> lock(device_memory_lock());
             HCHK(hipHostMalloc((void**)&slot.host_fi, FI_WORDS * sizeof(uint32_t), hipHostMallocDefault));
@@ -956,6 +987,23 @@ int fhtgs_tuned_digit_bits(fhtgs_handle h, int32_t width, int32_t height, int32_
     if (!h || !bits || width <= 0 || height <= 0) return FHTGS_E_INVALID;
     const int64_t ns = (int64_t)((width + kTileL - 1) / kTileL) * ((height + kTileL - 1) / kTileL) * kNumSub;
     *bits = digit_bits_for(h, ns);
+    return FHTGS_OK;
+}
+
+int fhtgs_build_tables(fhtgs_handle h, const uint8_t* visible, int32_t width, int32_t height,
+                       int32_t foveation_radius_px, int32_t blend_width_px, uint32_t* mask_out,
+                       uint32_t* vis_sat_out, uint32_t* fovea_sat_out, void* stream_ptr) {
+    if (!h) return FHTGS_E_INVALID;
+    if (width <= 0 || height <= 0) return set_err(h, FHTGS_E_INVALID, "width and height must be positive");
+    if (!mask_out && vis_sat_out) return set_err(h, FHTGS_E_INVALID, "the visibility SAT needs the mask output");
+    const int gw = (width + kTileL - 1) / kTileL, gh = (height + kTileL - 1) / kTileL;
+    if (gw * gh > 65536) return set_err(h, FHTGS_E_INVALID, "more than 65536 tiles");
+    int R = 0, B = 0;
+    fov_radii(width, height, foveation_radius_px, blend_width_px, R, B);
+    DeviceScope scope(h->device);
+    if (!scope.ok) return set_err(h, FHTGS_E_HIP, "cannot select the handle's device");
+    HCHK(launch_build_tables(visible, mask_out != nullptr, gw, gh, R, mask_out, vis_sat_out, fovea_sat_out,
+                             (hipStream_t)stream_ptr));
     return FHTGS_OK;
 }
 

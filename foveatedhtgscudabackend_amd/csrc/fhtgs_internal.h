@@ -94,6 +94,10 @@ struct ViewBuffers {
     ushort4* bounds;          // N
     float4* prim;             // 4N: one 64-B record per primitive (VPMT1, VPMT2, VPMT4, rgba)
     uint32_t* block_sums;     // ceil(N / kPrimBlock), then exclusive offsets
+    // culling tables built on the device when the view brings none
+    uint32_t* tab_mask;       // ceil(n_large / 32)
+    uint32_t* tab_vis_sat;    // (gw+1)*(gh+1)
+    uint32_t* tab_fovea_sat;  // (2R+1)^2
 };
 
 struct SortPlan {
@@ -145,6 +149,11 @@ hipError_t launch_clear_image(const ViewParams& vp, const ViewParams* dvp, const
                               hipStream_t s);
 hipError_t launch_widen_keys(const void* keys, int key_bytes, uint32_t* out, uint32_t n,
                              hipStream_t s);
+// Culling tables on the device. pack: build the mask `words` from `visible`
+// (gh*gw bytes, null = all visible); otherwise `words` is the input mask.
+// vis_sat / fovea_sat may be null.
+hipError_t launch_build_tables(const uint8_t* visible, bool pack, int gw, int gh, int R, uint32_t* words,
+                               uint32_t* vis_sat, uint32_t* fovea_sat, hipStream_t s);
 hipError_t launch_gaze_marker(float* img, int W, int H, float gaze_x, float gaze_y, const float rgb[3], int size,
                               int type, int chw, hipStream_t s);
 hipError_t launch_tile_borders(const ViewParams& vp, const ViewParams* dvp, const ViewBuffers& vb, float* img,

@@ -109,3 +109,30 @@ def fovea_radius_tiles_for(width: int, height: int, foveation_radius_px: int, bl
     if foveation_radius_px < 0:
         return int(np.ceil(np.sqrt(float(gw * gw + gh * gh)))) + 1
     return (foveation_radius_px + TILE - 1) // TILE
+
+
+def build_tables_device(width: int, height: int, visible: torch.Tensor | None = None,
+                        foveation_radius_px: int = _lib.DEFAULT_FOVEATION_RADIUS_PX,
+                        blend_width_px: int = _lib.DEFAULT_BLEND_WIDTH_PX, device="cuda"):
+    """The same three tables built by kernels on the GPU (fhtgs_build_tables):
+    ``visible`` is an optional (gh, gw) bool/uint8 GPU tensor (None = all
+    visible). Enqueued on the current stream; the result equals build_tables."""
+    gw, gh = grid_size(width, height)
+    R = fovea_radius_tiles_for(width, height, foveation_radius_px, blend_width_px)
+    dev = torch.device(device)
+    if dev.index is None:
+        dev = torch.device("cuda", torch.cuda.current_device())
+    dt = getattr(torch, "uint32", torch.int32)
+    mask = torch.empty((gw * gh + 31) // 32, dtype=dt, device=dev)
+    vsat = torch.empty((gw + 1) * (gh + 1), dtype=dt, device=dev)
+    fsat = torch.empty((2 * R + 1) ** 2, dtype=dt, device=dev)
+    vis_ptr = 0
+    if visible is not None:
+        if tuple(visible.shape) != (gh, gw) or visible.device.type != "cuda":
+            raise ValueError(f"visible must be a ({gh},{gw}) GPU tensor")
+        visible = visible.to(torch.uint8).contiguous()
+        vis_ptr = visible.data_ptr()
+    _lib.handle_for(dev.index).build_tables(vis_ptr, width, height, foveation_radius_px, blend_width_px,
+                                             mask.data_ptr(), vsat.data_ptr(), fsat.data_ptr(),
+                                             torch.cuda.current_stream(dev).cuda_stream)
+    return mask, vsat, fsat

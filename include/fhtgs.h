@@ -34,6 +34,9 @@
  *                                rasterization/include/utils/buffer_utils.h:10-143
  *   fhtgs_sort_pairs          <- cub::DeviceRadixSort::SortPairs over bits [0,end_bit),
  *                                rasterization/src/inference.cu:267-274
+ *   fhtgs_build_tables        <- the caller-side culling tables the reference takes as
+ *                                arguments (torch_bindings/rasterization.py:28-30;
+ *                                layouts read by kernel_utils.cuh:168-210)
  *   fhtgs_fovea_radius_tiles  <- config::foveation_radius_tiles / blend_radius_tiles,
  *                                rasterization/include/config.h:30-35 (made runtime)
  *   fhtgs_debug_read / fhtgs_debug_counts: stage outputs of the last render
@@ -96,9 +99,11 @@ typedef struct {
     const float* VPR_inv;      /* device, 4x4 row major, screen -> world directions */
     const float* cam_position; /* device, 3 */
     float gaze_x, gaze_y;      /* host values, pixels */
-    const uint32_t* visibility_mask;            /* device, ceil(n_large/32) words */
-    const uint32_t* visibility_mask_area_table; /* device, (gw+1)*(gh+1) */
-    const uint32_t* fovea_mask_area_table;      /* device, (2R+1)^2 */
+    const uint32_t* visibility_mask;            /* device, ceil(n_large/32) words; NULL with both
+                                                   tables NULL: all tiles visible */
+    const uint32_t* visibility_mask_area_table; /* device, (gw+1)*(gh+1); NULL (with the next
+                                                   also NULL): built in the frame */
+    const uint32_t* fovea_mask_area_table;      /* device, (2R+1)^2; NULL: built in the frame */
     const float* background_model_data;         /* device; SH: 48 floats, texture: (3,h,w) */
     int32_t background_model_type;
     int32_t background_width;  /* texture w */
@@ -167,6 +172,19 @@ int fhtgs_workspace_bytes(fhtgs_handle h, int64_t* bytes);
  * key_bytes 2 or 4, values uint32. Uses the handle's scratch. */
 int fhtgs_sort_pairs(fhtgs_handle h, void* keys, uint32_t* values, int64_t n,
                      int32_t key_bytes, int32_t end_bit, void* stream);
+
+/* Culling tables on the device, in the layouts the render reads (the
+ * reference takes them from its caller, torch_bindings/rasterization.py:28-30):
+ * mask_out, ceil(gw*gh/32) words, bit t%32 of word t/32 for 16-px tile t,
+ * from `visible` (gw*gh bytes, non-zero = visible; NULL = all visible);
+ * vis_sat_out, (gw+1)*(gh+1) entries, x+(gw+1)*y = visible tiles in
+ * [0,x)x[0,y) (needs mask_out); fovea_sat_out, (2R+1)^2 entries for R of
+ * fhtgs_fovea_radius_tiles. Any output may be NULL; enqueued on `stream`.
+ * A view whose two SAT pointers are both NULL gets them built this way inside
+ * its frame (and, with a NULL mask, an all-visible mask). */
+int fhtgs_build_tables(fhtgs_handle h, const uint8_t* visible, int32_t width, int32_t height,
+                       int32_t foveation_radius_px, int32_t blend_width_px, uint32_t* mask_out,
+                       uint32_t* vis_sat_out, uint32_t* fovea_sat_out, void* stream);
 
 /* Draws the gaze marker into `image` (device float32, 3*H*W, layout by
  * to_chw): the gaze clamped to the image and truncated to a pixel, a marker of
