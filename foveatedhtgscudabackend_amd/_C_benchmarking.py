@@ -14,12 +14,13 @@ def benchmark(positions, scales, rotations, opacities, sh_0, sh_rest,
               background_model_type, K, active_sh_bases, width, height, focal_x, focal_y,
               center_x, center_y, near_plane, far_plane, scale_modifier,
               to_chw, blur_periphery, anti_aliasing, *,
-              foveation_radius_px=DEFAULT_FOVEATION_RADIUS_PX, blend_width_px=DEFAULT_BLEND_WIDTH_PX):
+              foveation_radius_px=DEFAULT_FOVEATION_RADIUS_PX, blend_width_px=DEFAULT_BLEND_WIDTH_PX,
+              quarter_radius_px=0):
     view = (M, VPM, VPR_inv, cam_position, gaze_position, visibility_mask,
             visibility_mask_area_table, fovea_mask_area_table, background_model_data,
             background_model_type, K, active_sh_bases, width, height, focal_x, focal_y,
             center_x, center_y, near_plane, far_plane, scale_modifier)
     (image,) = render_views((positions, scales, rotations, opacities, sh_0, sh_rest), [view],
                             False, False, to_chw, blur_periphery, anti_aliasing,
-                            foveation_radius_px, blend_width_px)
+                            foveation_radius_px, blend_width_px, quarter_radius_px)
     return image

@@ -16,7 +16,7 @@ from ._lib import DEFAULT_BLEND_WIDTH_PX, DEFAULT_FOVEATION_RADIUS_PX
 
 def render(positions, scales, rotations, opacities, sh_0, sh_rest, *args,
            foveation_radius_px=DEFAULT_FOVEATION_RADIUS_PX, blend_width_px=DEFAULT_BLEND_WIDTH_PX,
-           clear_output=False):
+           clear_output=False, quarter_radius_px=0):
     if len(args) != 45:
         raise TypeError(f"_C_stereo.render expects 51 positional arguments, got {6 + len(args)}")
     left, right = list(args[0:21]), list(args[21:42])
@@ -27,5 +27,5 @@ def render(positions, scales, rotations, opacities, sh_0, sh_rest, *args,
         right[k] = left[k]
     images = render_views((positions, scales, rotations, opacities, sh_0, sh_rest),
                           [tuple(left), tuple(right)], True, clear_output, to_chw, blur_periphery,
-                          anti_aliasing, foveation_radius_px, blend_width_px)
+                          anti_aliasing, foveation_radius_px, blend_width_px, quarter_radius_px)
     return images[0], images[1]

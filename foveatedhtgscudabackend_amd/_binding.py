@@ -63,7 +63,7 @@ def view_struct(device, M, VPM, VPR_inv, cam_position, gaze_position, visibility
                 background_model_type, K, active_sh_bases, width, height, focal_x, focal_y,
                 center_x, center_y, near_plane, far_plane, scale_modifier, to_chw,
                 blur_periphery, anti_aliasing, foveation_radius_px, blend_width_px,
-                clear_output, keep: list) -> tuple:
+                clear_output, keep: list, quarter_radius_px: int = 0) -> tuple:
     v = _lib.View()
     width, height = int(width), int(height)
     if M.numel() < 12:
@@ -130,6 +130,7 @@ def view_struct(device, M, VPM, VPR_inv, cam_position, gaze_position, visibility
     v.anti_aliasing = 1 if anti_aliasing else 0
     v.foveation_radius_px = int(foveation_radius_px)
     v.blend_width_px = int(blend_width_px)
+    v.quarter_radius_px = int(quarter_radius_px)
     v.clear_output = 1 if clear_output else 0
     shape = (3, height, width) if to_chw else (height, width, 3)
     image = torch.empty(shape, dtype=torch.float32, device=device)
@@ -142,13 +143,14 @@ def view_struct(device, M, VPM, VPR_inv, cam_position, gaze_position, visibility
 
 
 def render_views(scene_args: tuple, view_args: list, share_preprocess: bool, clear_output: bool,
-                 to_chw, blur_periphery, anti_aliasing, foveation_radius_px, blend_width_px):
+                 to_chw, blur_periphery, anti_aliasing, foveation_radius_px, blend_width_px,
+                 quarter_radius_px: int = 0):
     keep: list = []
     scene, device = scene_struct(*scene_args, keep=keep)
     views, images = [], []
     for va in view_args:
         v, img = view_struct(device, *va, to_chw, blur_periphery, anti_aliasing, foveation_radius_px,
-                             blend_width_px, clear_output, keep=keep)
+                             blend_width_px, clear_output, keep=keep, quarter_radius_px=quarter_radius_px)
         views.append(v)
         images.append(img)
     handle = _lib.handle_for(device.index if device.index is not None else torch.cuda.current_device())

@@ -28,7 +28,7 @@ DBG = {
     "tile_num": 1, "tile_map": 2, "tile_type": 3, "tile_map_part": 4, "partition": 6,
     "n_touched": 7, "bounds": 8, "vpmt1": 9, "vpmt2": 10, "vpmt4": 11, "rgba": 12,
     "keys_unsorted": 14, "vals_unsorted": 15, "keys_sorted": 16, "vals_sorted": 17,
-    "ranges": 18,
+    "ranges": 18, "partition_quarter": 19,
 }
 OPT_KEEP_UNSORTED = 1
 OPT_SORT_DIGIT_BITS = 2
@@ -77,6 +77,7 @@ class View(ctypes.Structure):
         ("anti_aliasing", ctypes.c_int32), ("foveation_radius_px", ctypes.c_int32),
         ("blend_width_px", ctypes.c_int32), ("clear_output", ctypes.c_int32),
         ("image", ctypes.c_void_p), ("image_temp", ctypes.c_void_p),
+        ("quarter_radius_px", ctypes.c_int32), ("reserved", ctypes.c_int32),
     ]
 
 

@@ -29,6 +29,7 @@ class BatchView:
     foveation_radius_px: int = _lib.DEFAULT_FOVEATION_RADIUS_PX
     blend_width_px: int = _lib.DEFAULT_BLEND_WIDTH_PX
     clear_output: bool = True
+    quarter_radius_px: int = 0  # extension: quarter-resolution level beyond this radius, 0 = off
     slot: int | None = None  # handle slot (persistent buffers), default: position in the batch
 
 
@@ -49,7 +50,7 @@ def _marshal(scenes: list, views: list):
             raise ValueError(f"view refers to scene {bv.scene} of {len(scenes)}")
         v, img = _binding.view_struct(device, *bv.settings.as_tuple(), bv.to_chw, bv.blur_periphery,
                                       bv.anti_aliasing, bv.foveation_radius_px, bv.blend_width_px,
-                                      bv.clear_output, keep=keep)
+                                      bv.clear_output, keep=keep, quarter_radius_px=bv.quarter_radius_px)
         vstructs.append(v)
         images.append(img)
         index.append(bv.scene)

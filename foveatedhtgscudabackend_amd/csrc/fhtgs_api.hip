@@ -271,6 +271,12 @@ int validate_view(fhtgs_context* h, const fhtgs_view& v, int idx) {
         return set_err(h, FHTGS_E_INVALID, p + "texture background needs its size");
     if (v.foveation_radius_px >= 0 && v.blend_width_px > v.foveation_radius_px)
         return set_err(h, FHTGS_E_INVALID, p + "blend width exceeds the foveation radius");
+    if (v.quarter_radius_px > 0) {
+        if (v.foveation_radius_px < 0 || v.quarter_radius_px < v.foveation_radius_px)
+            return set_err(h, FHTGS_E_INVALID, p + "the quarter radius needs foveation and must be >= its radius");
+        if (v.blur_periphery)
+            return set_err(h, FHTGS_E_UNSUPPORTED, p + "the quarter level does not combine with blur_periphery");
+    }
     return FHTGS_OK;
 }
 
@@ -296,6 +302,7 @@ void fill_params(const fhtgs_scene& sc, const fhtgs_view& v, ViewParams& vp) {
     fov_radii(v.width, v.height, v.foveation_radius_px, v.blend_width_px, vp.fovea_r, vp.blend_r);
     vp.blend_radius_px = vp.blend_r * kTileL;
     vp.blend_width_px = v.blend_width_px;
+    vp.quarter_r = v.quarter_radius_px > 0 ? (v.quarter_radius_px + kTileL - 1) / kTileL : 0;
     // rasterization/src/inference.cu:79-105: clamp, then round to a large
     // tile; the clamp bound is the small-grid size, as in the reference
     vp.gcx = std::fmax(0.0f, std::fmin(v.gaze_x, (float)(v.width - 1)));
@@ -441,8 +448,12 @@ int enqueue_phase2(fhtgs_context* h, const fhtgs_view* views, int n_views, bool 
     for (int v = 0; v < n_views; ++v) {
         ViewSlot& slot = (*h->cur[v]);
         if (views[v].clear_output) HCHK(launch_clear_image(slot.vp, slot.dvp, slot.vb, s));
-        HCHK(launch_blend(slot.dvp, slot.vb, slot.ib.vals[slot.sorted_src], views[v].K,
-                          exact ? slot.n_active : (uint32_t)slot.vp.n_small, h->xcd_run, s));
+        uint32_t grid = (uint32_t)slot.vp.n_small;
+        if (exact) {  // QUARTER tiles (the last partition) go four to a workgroup
+            const uint32_t nq = slot.host_fi[FI_PART + 7] - slot.host_fi[FI_PART + 6];
+            grid = slot.n_active - nq + (nq + 3u) / 4u;
+        }
+        HCHK(launch_blend(slot.dvp, slot.vb, slot.ib.vals[slot.sorted_src], views[v].K, grid, h->xcd_run, s));
     }
     if (timed && (rc = record(h, 9, s))) return rc;
     for (int v = 0; v < n_views; ++v) {
@@ -450,7 +461,7 @@ int enqueue_phase2(fhtgs_context* h, const fhtgs_view* views, int n_views, bool 
         uint32_t grid = (uint32_t)slot.vp.n_large;
         if (exact) {
             const uint32_t* part = slot.host_fi + FI_PART;
-            grid = (part[3] - part[2]) + (part[5] - part[4] + 3u) / 4u;
+            grid = (part[3] - part[2]) + (part[5] - part[4] + 3u) / 4u + (part[7] - part[6]);
         }
         HCHK(launch_upsample(slot.dvp, slot.vb, grid, s));
     }
@@ -1096,6 +1107,7 @@ int fhtgs_debug_read(fhtgs_handle h, int32_t view, int32_t id, void* dst, int64_
         case FHTGS_DBG_TILE_TYPE: src = sl.vb.tile_type; bytes = na; break;
         case FHTGS_DBG_TILE_MAP_PART: src = sl.vb.tile_map_part; bytes = na * 4; break;
         case FHTGS_DBG_PARTITION: src = sl.vb.frame_info + FI_PART; bytes = 24; break;
+        case FHTGS_DBG_PARTITION_QUARTER: src = sl.vb.frame_info + FI_PART + 6; bytes = 8; break;
         case FHTGS_DBG_N_TOUCHED: src = sl.vb.n_touched; bytes = N * 4; break;
         case FHTGS_DBG_BOUNDS: src = sl.vb.bounds; bytes = N * 8; break;
         case FHTGS_DBG_VPMT1: src = sl.vb.prim + 0; bytes = N * 16; strided = true; break;

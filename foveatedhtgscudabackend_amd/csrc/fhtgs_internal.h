@@ -21,6 +21,10 @@ constexpr int kNumSub = 4;
 constexpr int kTileFovea = 0;
 constexpr int kTilePeriphery = 1;
 constexpr int kTileBlended = 2;
+// Extension: PERIPHERY tiles beyond the quarter radius are rendered at stride
+// 4 (4x4 samples per 16x16 tile, four tiles per wave) and upsampled 4x.
+constexpr int kTileQuarter = 3;
+constexpr int kQuarterStride = 4;
 
 // Sort geometry: 256 threads x 16 keys per onesweep tile.
 constexpr int kSortThreads = 256;
@@ -41,12 +45,13 @@ constexpr int kPrimBlock = 256;  // preprocess / instance block (shared partitio
 
 // Frame-info words (device, one block per view), read back once per frame.
 enum FrameInfo : int {
-    FI_PART = 0,          // 6 words: (x,y) per tile type
-    FI_N_ACTIVE = 6,
-    FI_INSTANCES = 7,     // I = sum of tiles-touched
-    FI_VALID = 8,         // non-sentinel instances (atomic)
-    FI_ERROR = 9,         // bit 0: tiles-touched under-count, bit 2: instance capacity exceeded
-    FI_N_VISIBLE = 10,    // preprocess survivors (atomic)
+    FI_PART = 0,          // 8 words: (x,y) per tile type, fovea, periphery, blended, quarter
+    FI_N_ACTIVE = 8,
+    FI_INSTANCES = 9,     // I = sum of tiles-touched
+    FI_VALID = 10,        // non-sentinel instances (atomic)
+    FI_ERROR = 11,        // bit 0: tiles-touched under-count, bit 1: sort inconsistency,
+                          // bit 2: instance capacity exceeded
+    FI_N_VISIBLE = 12,    // preprocess survivors (atomic)
     FI_WORDS = 16
 };
 
@@ -62,6 +67,7 @@ struct ViewParams {
     int bg_type, bg_w, bg_h;
     int width, height, gw, gh, n_large, n_small;
     int fovea_r, blend_r, blend_radius_px, blend_width_px;
+    int quarter_r;        // quarter-resolution radius in large tiles; 0 = no quarter level
     int gtx, gty;
     float gcx, gcy;
     int active_sh, total_sh;

@@ -7,6 +7,9 @@
 // pixel the K nearest fragments with alpha >= 0.05 are kept sorted in
 // registers with 8-bit quantised colour, everything else is accumulated
 // order-independently in the tail; resolve, optional background, saturate.
+// Extension: QUARTER tiles cover their 16x16 large tile at stride 4 (4x4
+// samples), four tiles per wave: 16-lane groups, each walking its own tile's
+// range in batches of 16 through its own quarter of the LDS batch.
 #include "fhtgs_device.h"
 #include "fhtgs_cull.h"
 
@@ -139,25 +142,38 @@ __global__ __launch_bounds__(64) void blend_kernel(const ViewParams* vpp, BlendA
     __shared__ float4 sV1[64], sV2[64], sV4[64], sC[64];
     const uint32_t g = xcd_remap(blockIdx.x, gridDim.x, a.xcd_run);
     const uint32_t* fi = a.frame_info;
+    const uint32_t lane = threadIdx.x;
     int type;
+    uint32_t j = g;  // this lane's partitioned tile
     if (g >= fi[0] && g < fi[1]) type = kTileFovea;
     else if (g >= fi[2] && g < fi[3]) type = kTilePeriphery;
     else if (g >= fi[4] && g < fi[5]) type = kTileBlended;
-    else return;
-    const uint32_t tg = a.tile_map_part[g];
+    else if (g >= fi[6] && g < fi[6] + (fi[7] - fi[6] + 3u) / 4u) {
+        type = kTileQuarter;
+        j = fi[6] + (g - fi[6]) * 4u + (lane >> 4);
+    } else return;
+    // lanes form groups of `gsz` with a shared tile and range: the whole wave,
+    // or 16 lanes per QUARTER tile (the last wave may have empty groups)
+    const bool quarter = type == kTileQuarter;
+    const uint32_t gsz = quarter ? 16u : 64u;
+    const uint32_t gbase = quarter ? (lane & ~15u) : 0u;
+    const uint32_t li = lane - gbase;
+    const uint64_t gmask = quarter ? (0xffffull << gbase) : ~0ull;
+    const bool have = !quarter || j < fi[7];
+    const uint32_t tg = have ? a.tile_map_part[j] : 0u;
     const uint32_t lt = tg / kNumSub, st = tg % kNumSub;
     const uint32_t ltx = lt % (uint32_t)vp.gw, lty = lt / (uint32_t)vp.gw;
-    const uint32_t lane = threadIdx.x;
-    const uint32_t step = type == kTilePeriphery ? kStride : 1u;
+    const uint32_t step = quarter ? (uint32_t)kQuarterStride : (type == kTilePeriphery ? kStride : 1u);
+    const uint32_t cols = quarter ? 4u : 8u;  // samples per row
     const uint32_t bx = ltx * kTileL + (st % kStride) * kTileS;
     const uint32_t by = lty * kTileL + (st / kStride) * kTileS;
-    const uint32_t px = bx + (lane & 7u) * step;
-    const uint32_t py = by + (lane >> 3) * step;
-    const bool inside = px < (uint32_t)vp.width && py < (uint32_t)vp.height;
+    const uint32_t px = bx + (li % cols) * step;
+    const uint32_t py = by + (li / cols) * step;
+    const bool inside = have && px < (uint32_t)vp.width && py < (uint32_t)vp.height;
     const float pxf = (float)px, pyf = (float)py;
-    const float tx0 = (float)bx, tx1 = (float)(bx + 7u * step);
-    const float ty0 = (float)by, ty1 = (float)(by + 7u * step);
-    const uint64_t lt_mask = lanemask_lt();
+    const float tx0 = (float)bx, tx1 = (float)(bx + (cols - 1u) * step);
+    const float ty0 = (float)by, ty1 = (float)(by + (cols - 1u) * step);
+    const uint64_t lt_mask = lanemask_lt() & gmask;
 
     float cd[K];
     uint32_t cc[K];
@@ -166,12 +182,13 @@ __global__ __launch_bounds__(64) void blend_kernel(const ViewParams* vpp, BlendA
     float T_tail = 1.0f;
     float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f, t3 = 0.0f;
 
-    const uint2 range = a.ranges[tg];
-    for (uint32_t b0 = range.x; b0 < range.y; b0 += 64u) {
-        // each lane loads one instance of the next 64 and tests its alpha
-        // support ellipse against the tile's samples (fh_tile_may_touch);
-        // survivors are compacted in order into the LDS batch
-        const uint32_t idx = b0 + lane;
+    const uint2 range = have ? a.ranges[tg] : make_uint2(0u, 0u);
+    for (uint32_t b0 = range.x; __ballot(b0 < range.y) != 0ull; b0 += gsz) {
+        // each lane loads one instance of its group's next gsz and tests its
+        // alpha support ellipse against the tile's samples
+        // (fh_tile_may_touch); survivors are compacted in order into the
+        // group's part of the LDS batch
+        const uint32_t idx = b0 + li;
         bool keep = false;
         float4 V1, V2, V4, C;
         if (idx < range.y) {
@@ -186,11 +203,11 @@ __global__ __launch_bounds__(64) void blend_kernel(const ViewParams* vpp, BlendA
             keep = fh_tile_may_touch(a1, a2, a4, C.w, tx0, tx1, ty0, ty1);
         }
         const uint64_t km = __ballot(keep);
-        const int count = (int)__popcll(km);
-        if (count == 0) continue;
+        if (km == 0ull) continue;
+        const int count = (int)__popcll(km & gmask);
         __syncthreads();
         if (keep) {
-            const uint32_t pos = (uint32_t)__popcll(km & lt_mask);
+            const uint32_t pos = gbase + (uint32_t)__popcll(km & lt_mask);
             sV1[pos] = V1;
             sV2[pos] = V2;
             sV4[pos] = V4;
@@ -198,8 +215,9 @@ __global__ __launch_bounds__(64) void blend_kernel(const ViewParams* vpp, BlendA
         }
         __syncthreads();
         if (!inside) continue;
-        for (int j = 0; j < count; ++j) {
-            const float4 W1 = sV1[j], W2 = sV2[j], W4 = sV4[j];
+        for (int jj = 0; jj < count; ++jj) {
+            const uint32_t e = gbase + (uint32_t)jj;
+            const float4 W1 = sV1[e], W2 = sV2[e], W4 = sV4[e];
             const float Px0 = fmaf(-W4.x, pxf, W1.x), Px1 = fmaf(-W4.y, pxf, W1.y);
             const float Px2 = fmaf(-W4.z, pxf, W1.z), Px3 = fmaf(-W4.w, pxf, W1.w);
             const float Py0 = fmaf(-W4.x, pyf, W2.x), Py1 = fmaf(-W4.y, pyf, W2.y);
@@ -213,7 +231,7 @@ __global__ __launch_bounds__(64) void blend_kernel(const ViewParams* vpp, BlendA
             // num <= 11.08 den here, so the exponent lies in [-6, 0] (or is NaN)
             const float rcp = rcp_rn(den);
             const float G = fh_expf_m64((-0.5f * num) * rcp);
-            const float4 Cj = sC[j];
+            const float4 Cj = sC[e];
             const float alpha = fminf(Cj.w * G, kMaxFragmentAlpha);
             if (alpha < kMinAlpha) continue;
             float p0 = Cj.x * alpha, p1 = Cj.y * alpha, p2 = Cj.z * alpha, p3 = alpha;
@@ -296,8 +314,9 @@ __global__ __launch_bounds__(64) void blend_kernel(const ViewParams* vpp, BlendA
     }
 }
 
-// `grid` is the active tile count, or n_small as a bound: workgroups outside
-// the three partition ranges return at once.
+// `grid` is the active tile count with QUARTER tiles counted four to a
+// workgroup, or n_small as a bound: workgroups outside the partition ranges
+// return at once.
 hipError_t launch_blend(const ViewParams* dvp, const ViewBuffers& vb, const uint32_t* sorted_vals,
                         int K, uint32_t grid, int xcd_run, hipStream_t s) {
     if (grid == 0) return hipSuccess;

@@ -8,6 +8,13 @@
 //     (interpolation.cuh:149-267, 416-584).
 // Samples are always even/even pixels, which this launch never writes, so the
 // in-place variant has no read/write overlap between workgroups.
+// Extension: QUARTER large tiles (one workgroup each) fill their 16x16 pixels
+// bilinearly from the 5x5 stride-4 samples (fractions 0, 1/4, 1/2, 3/4) of
+// the tile and its right and lower neighbours. A QUARTER tile's even/even
+// pixels are not all samples, so the stride-2 grids treat QUARTER tiles like
+// invisible ones (clamped into their own tile); stride-4 positions of every
+// visible tile are written by the blend only, so the quarter grid reads any
+// visible neighbour.
 #include "fhtgs_device.h"
 
 namespace fhtgs {
@@ -33,17 +40,26 @@ __device__ __forceinline__ void write_rgb(float* img, int x, int y, int W, int H
     }
 }
 
-// One stride-2 sample of the grid starting at (bx, by); samples in invisible
-// large tiles are clamped into the tile (lx, ly) (interpolation.cuh:70-101).
+// The tile-type rule of the tiles kernel: a visible large tile is QUARTER
+// when its centre lies at least quarter_r tiles from the gaze tile.
+__device__ __forceinline__ bool quarter_tile(const ViewParams& vp, int tile) {
+    return vp.quarter_r > 0 && !in_fovea(vp.quarter_r, (float)(tile % vp.gw) + 0.5f,
+                                         (float)(tile / vp.gw) + 0.5f, (float)vp.gtx, (float)vp.gty);
+}
+
+// One sample of the stride-S grid starting at (bx, by); samples in invisible
+// large tiles -- and, for S = 2, in QUARTER tiles -- are clamped into the
+// tile (lx, ly) (interpolation.cuh:70-101).
+template <int S>
 __device__ __forceinline__ v3 grid_sample(const ViewParams& vp, const float* img, int lx, int ly,
                                           int bx, int by, int sx, int sy) {
-    const int mx = (vp.width - bx - 1) / kStride, my = (vp.height - by - 1) / kStride;
-    const int px = bx + min(sx, mx) * kStride;
-    const int py = by + min(sy, my) * kStride;
+    const int mx = (vp.width - bx - 1) / S, my = (vp.height - by - 1) / S;
+    const int px = bx + min(sx, mx) * S;
+    const int py = by + min(sy, my) * S;
     const int tile = (py / kTileL) * vp.gw + px / kTileL;
-    if (!mask_bit(vp.vis_mask, (uint32_t)tile)) {
-        const int cx = max(lx * kTileL, min(lx * kTileL + kTileL - kStride, px));
-        const int cy = max(ly * kTileL, min(ly * kTileL + kTileL - kStride, py));
+    if (!mask_bit(vp.vis_mask, (uint32_t)tile) || (S == kStride && quarter_tile(vp, tile))) {
+        const int cx = max(lx * kTileL, min(lx * kTileL + kTileL - S, px));
+        const int cy = max(ly * kTileL, min(ly * kTileL + kTileL - S, py));
         return read_rgb(img, cx, cy, vp.width, vp.height, vp.to_chw);
     }
     return read_rgb(img, px, py, vp.width, vp.height, vp.to_chw);
@@ -56,9 +72,11 @@ __device__ __forceinline__ void acc_add(v3& a, v3 s) {
     a.x = a.x + s.x; a.y = a.y + s.y; a.z = a.z + s.z;
 }
 
-// interpolation.cuh:112-132 (frac in {0, 1/2})
+// interpolation.cuh:112-132 (frac in {0, 1/2}); for the stride-4 QUARTER
+// grid the same weights with frac in {0, 1/4, 1/2, 3/4}
+template <int S>
 __device__ __forceinline__ v3 bilinear(const v3* s, int nx, int bsx, int bsy, int xm, int ym) {
-    const float xf = (float)xm / 2.0f, yf = (float)ym / 2.0f;
+    const float xf = (float)xm / (float)S, yf = (float)ym / (float)S;
     v3 acc{0.0f, 0.0f, 0.0f};
     if (xm == 0) {
         acc_fma(acc, s[bsy * nx + bsx], 1.0f - yf);
@@ -126,7 +144,7 @@ __global__ __launch_bounds__(256) void upsample_kernel(const ViewParams* vpp, co
         const int off = vp.blur ? kStride : 0;
         const int bx = lx * kTileL - off, by = ly * kTileL - off;
         for (int k = tid; k < ns * ns; k += 256)
-            s_smp[k] = grid_sample(vp, src, lx, ly, bx, by, k % ns, k / ns);
+            s_smp[k] = grid_sample<kStride>(vp, src, lx, ly, bx, by, k % ns, k / ns);
         __syncthreads();
         const int tx = tid & 15, ty = tid >> 4;
         const int px = lx * kTileL + tx, py = ly * kTileL + ty;
@@ -135,7 +153,7 @@ __global__ __launch_bounds__(256) void upsample_kernel(const ViewParams* vpp, co
             const int xm = tx % kStride, ym = ty % kStride;
             if (xm == 0 && ym == 0) return;
             write_rgb(vp.image_final, px, py, W, H, vp.to_chw,
-                      sat3(bilinear(s_smp, ns, tx / kStride, ty / kStride, xm, ym)));
+                      sat3(bilinear<kStride>(s_smp, ns, tx / kStride, ty / kStride, xm, ym)));
         } else {
             v3 acc;
             const float f = blur_sum(s_smp, ns, tx / 2 + 1, ty / 2 + 1, tx, ty, acc);
@@ -143,8 +161,29 @@ __global__ __launch_bounds__(256) void upsample_kernel(const ViewParams* vpp, co
         }
         return;
     }
-    // BLENDED: four small tiles per workgroup
     const uint32_t nb = frame_info[FI_PART + 5] - frame_info[FI_PART + 4];
+    const uint32_t nb_groups = (nb + 3u) / 4u;
+    if (blockIdx.x >= n_periphery + nb_groups) {
+        // QUARTER: one large tile per workgroup
+        const uint32_t j = blockIdx.x - n_periphery - nb_groups;
+        if (j >= frame_info[FI_PART + 7] - frame_info[FI_PART + 6]) return;
+        const uint32_t lt = tile_map_part[frame_info[FI_PART + 6] + j] / kNumSub;
+        const int lx = (int)(lt % (uint32_t)vp.gw), ly = (int)(lt / (uint32_t)vp.gw);
+        constexpr int ns = kTileL / kQuarterStride + 1;
+        const int bx = lx * kTileL, by = ly * kTileL;
+        for (int k = tid; k < ns * ns; k += 256)
+            s_smp[k] = grid_sample<kQuarterStride>(vp, vp.image_final, lx, ly, bx, by, k % ns, k / ns);
+        __syncthreads();
+        const int tx = tid & 15, ty = tid >> 4;
+        const int px = bx + tx, py = by + ty;
+        if (px >= W || py >= H) return;
+        const int xm = tx % kQuarterStride, ym = ty % kQuarterStride;
+        if (xm == 0 && ym == 0) return;
+        write_rgb(vp.image_final, px, py, W, H, vp.to_chw,
+                  sat3(bilinear<kQuarterStride>(s_smp, ns, tx / kQuarterStride, ty / kQuarterStride, xm, ym)));
+        return;
+    }
+    // BLENDED: four small tiles per workgroup
     const int q = tid >> 6, l = tid & 63;
     const uint32_t j = (blockIdx.x - n_periphery) * 4u + (uint32_t)q;
     const bool have = j < nb;
@@ -159,7 +198,7 @@ __global__ __launch_bounds__(256) void upsample_kernel(const ViewParams* vpp, co
         oy = ly * kTileL + (int)(st / kStride) * kTileS;
         const int off = vp.blur ? kStride : 0;
         for (int k = l; k < ns * ns; k += 64)
-            smp[k] = grid_sample(vp, src, lx, ly, ox - off, oy - off, k % ns, k / ns);
+            smp[k] = grid_sample<kStride>(vp, src, lx, ly, ox - off, oy - off, k % ns, k / ns);
     }
     __syncthreads();
     if (!have) return;
@@ -169,7 +208,7 @@ __global__ __launch_bounds__(256) void upsample_kernel(const ViewParams* vpp, co
     if (!vp.blur) {
         const int xm = tx % kStride, ym = ty % kStride;
         if (xm == 0 && ym == 0) return;
-        const v3 ip = bilinear(smp, ns, tx / kStride, ty / kStride, xm, ym);
+        const v3 ip = bilinear<kStride>(smp, ns, tx / kStride, ty / kStride, xm, ym);
         const float bf = ring_factor(vp, px, py);
         const v3 ctr = read_rgb(vp.image_final, px, py, W, H, vp.to_chw);
         if (bf > 0.0f) {
@@ -195,8 +234,9 @@ __global__ __launch_bounds__(256) void upsample_kernel(const ViewParams* vpp, co
 }
 
 // One workgroup per PERIPHERY large tile, then one per four BLENDED small
-// tiles (a blended large tile's four); `grid` is that count, or n_large as a
-// bound -- workgroups past the partition find no tile and return.
+// tiles (a blended large tile's four), then one per QUARTER large tile;
+// `grid` is that count, or n_large as a bound -- workgroups past the
+// partition find no tile and return.
 hipError_t launch_upsample(const ViewParams* dvp, const ViewBuffers& vb, uint32_t grid, hipStream_t s) {
     if (grid == 0) return hipSuccess;
     upsample_kernel<<<grid, 256, 0, s>>>(dvp, vb.frame_info, vb.tile_map_part);

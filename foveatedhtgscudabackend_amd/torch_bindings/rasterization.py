@@ -80,16 +80,20 @@ class RasterizerSettings(NamedTuple):
 class FoveatedHTGSRasterizer(torch.nn.Module):
     """``foveation_radius_px`` / ``blend_width_px`` default to the reference's
     compile-time constants (375 / 125 px); ``foveation_radius_px=-1`` renders
-    every visible tile at full resolution."""
+    every visible tile at full resolution. ``quarter_radius_px`` > 0 (an
+    extension, default off) renders tiles beyond that radius at quarter
+    resolution."""
 
     def __init__(self, foveation_radius_px: int = DEFAULT_FOVEATION_RADIUS_PX,
-                 blend_width_px: int = DEFAULT_BLEND_WIDTH_PX):
+                 blend_width_px: int = DEFAULT_BLEND_WIDTH_PX, quarter_radius_px: int = 0):
         super().__init__()
         self.foveation_radius_px = int(foveation_radius_px)
         self.blend_width_px = int(blend_width_px)
+        self.quarter_radius_px = int(quarter_radius_px)
 
     def _fov(self) -> dict:
-        return {"foveation_radius_px": self.foveation_radius_px, "blend_width_px": self.blend_width_px}
+        return {"foveation_radius_px": self.foveation_radius_px, "blend_width_px": self.blend_width_px,
+                "quarter_radius_px": self.quarter_radius_px}
 
     def render(
             self,

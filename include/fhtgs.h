@@ -122,6 +122,13 @@ typedef struct {
     int32_t clear_output;        /* 1: pixels no tile writes read 0 (render); 0: left as is (benchmark) */
     float* image;                /* device float32, 3*H*W, layout by to_chw */
     float* image_temp;           /* device float32, 3*H*W; required iff blur_periphery */
+    /* Extension (no reference counterpart): tiles whose centre lies at least
+     * this many pixels (rounded up to 16-px tiles) from the gaze tile are
+     * rendered at quarter resolution and upsampled 4x; <= 0 = off (the
+     * reference's two levels). Must be >= foveation_radius_px; not with
+     * blur_periphery. */
+    int32_t quarter_radius_px;
+    int32_t reserved;
 } fhtgs_view;
 
 int fhtgs_create(int device, fhtgs_handle* out);
@@ -224,6 +231,7 @@ int fhtgs_fovea_radius_tiles(int32_t width, int32_t height, int32_t foveation_ra
 #define FHTGS_DBG_KEYS_SORTED 16
 #define FHTGS_DBG_VALS_SORTED 17
 #define FHTGS_DBG_RANGES 18         /* u32[2*n_small] */
+#define FHTGS_DBG_PARTITION_QUARTER 19 /* u32[2]: the quarter tiles' partition range */
 int fhtgs_debug_read(fhtgs_handle h, int32_t view, int32_t id, void* host_dst,
                      int64_t dst_bytes, int64_t* bytes_written);
 

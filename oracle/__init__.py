@@ -44,7 +44,7 @@ DTYPES = {
 }
 COUNT_NAMES = ("N", "n_large", "n_small", "n_active", "instances", "valid_instances", "end_bit",
                "key_bytes", "n_visible", "gw", "gh", "fovea_tiles", "blend_tiles", "gaze_tx",
-               "gaze_ty", "evaluations", "pass_cutoff", "pass_alpha", "core_inserts", "reserved")
+               "gaze_ty", "evaluations", "pass_cutoff", "pass_alpha", "core_inserts", "quarter_tiles")
 STAGES = ("tiles", "preprocess", "scan", "instances", "sort", "ranges", "blend", "upsample")
 
 
@@ -68,7 +68,7 @@ class OrcView(ctypes.Structure):
                 ("scale_modifier", ctypes.c_float), ("to_chw", ctypes.c_int32),
                 ("blur_periphery", ctypes.c_int32), ("anti_aliasing", ctypes.c_int32),
                 ("foveation_radius_px", ctypes.c_int32), ("blend_width_px", ctypes.c_int32),
-                ("blend_tile_stride", ctypes.c_int32)]
+                ("blend_tile_stride", ctypes.c_int32), ("quarter_radius_px", ctypes.c_int32)]
 
 
 _lib = None
@@ -155,6 +155,7 @@ def run(scene: dict, view: dict, stop_after: int = 0, want=None) -> dict:
     (uint32 arrays), width, height, focal_x, focal_y, center_x, center_y,
     near_plane, far_plane and optional K, active_sh_bases, scale_modifier,
     to_chw, blur_periphery, foveation_radius_px, blend_width_px,
+    quarter_radius_px (extension, 0 = off),
     background_model_type / background_model_data."""
     lib = load()
     keep = []
@@ -197,6 +198,7 @@ def run(scene: dict, view: dict, stop_after: int = 0, want=None) -> dict:
     v.foveation_radius_px = int(view.get("foveation_radius_px", 375))
     v.blend_width_px = int(view.get("blend_width_px", 125))
     v.blend_tile_stride = int(view.get("blend_tile_stride", 1))
+    v.quarter_radius_px = int(view.get("quarter_radius_px", 0))
     err = ctypes.create_string_buffer(512)
     r = lib.orc_run(ctypes.byref(sc), ctypes.byref(v), int(stop_after), err, 512)
     if not r:

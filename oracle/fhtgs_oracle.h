@@ -56,6 +56,7 @@ typedef struct {
     int32_t foveation_radius_px;     /* reference: 375; <0 means foveation off */
     int32_t blend_width_px;          /* reference: 125 */
     int32_t blend_tile_stride;       /* >1: blend/upsample only every n-th active tile (timing samples) */
+    int32_t quarter_radius_px;       /* extension: quarter-resolution level beyond this radius; <=0 off */
 } orc_view;
 
 /* Result arrays, fetched by id after orc_run. */
@@ -66,7 +67,7 @@ enum {
     ORC_TILE_TYPE = 3,       /* u8 [n_active] */
     ORC_TILE_MAP_PART = 4,   /* u32[n_active] after the stable type partition */
     ORC_TILE_TYPE_PART = 5,  /* u8 [n_active] */
-    ORC_PARTITION = 6,       /* u32[6] (x,y) per type FOVEA, PERIPHERY, BLENDED */
+    ORC_PARTITION = 6,       /* u32[8] (x,y) per type FOVEA, PERIPHERY, BLENDED, QUARTER */
     ORC_N_TOUCHED = 7,       /* u32[N] */
     ORC_BOUNDS = 8,          /* u16[4N] (xmin,xmax,ymin,ymax), valid where n_touched>0 */
     ORC_VPMT1 = 9,           /* f32[4N] */
@@ -91,7 +92,7 @@ enum {
     ORC_C_KEY_BYTES = 7, ORC_C_N_VISIBLE = 8, ORC_C_GRID_W = 9, ORC_C_GRID_H = 10,
     ORC_C_FOVEA_R_TILES = 11, ORC_C_BLEND_R_TILES = 12, ORC_C_GAZE_TX = 13,
     ORC_C_GAZE_TY = 14, ORC_C_EVALUATIONS = 15, ORC_C_PASS_CUTOFF = 16, ORC_C_PASS_ALPHA = 17,
-    ORC_C_CORE_INSERTS = 18
+    ORC_C_CORE_INSERTS = 18, ORC_C_QUARTER_R_TILES = 19
 };
 
 typedef struct orc_result orc_result;

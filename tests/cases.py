@@ -30,6 +30,7 @@ class Case:
     background: int = 0              # 0 none, 1 SH, 2 texture, 3 raw texture
     scale_modifier: float = 1.0
     aaa: bool = False
+    quarter_radius_px: int = 0       # quarter-resolution level beyond this radius (0 = off)
     extra: dict = field(default_factory=dict)
 
     def build(self):
@@ -59,7 +60,8 @@ class Case:
         import oracle
         kw = dict(K=self.K, active_sh_bases=self.active_sh, to_chw=self.to_chw, blur_periphery=self.blur,
                   foveation_radius_px=self.foveation_radius_px, blend_width_px=self.blend_width_px,
-                  scale_modifier=self.scale_modifier, anti_aliasing=self.aaa)
+                  scale_modifier=self.scale_modifier, anti_aliasing=self.aaa,
+                  quarter_radius_px=self.quarter_radius_px)
         if bg is not None:
             kw["background_model_type"] = self.background
             kw["background_model_data"] = bg

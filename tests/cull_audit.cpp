@@ -47,29 +47,32 @@ int main(int argc, char** argv) {
     const int64_t N = h[0], I = h[1], n_small = h[2], n_active = h[3], gw = h[4], W = h[5], H = h[6];
     std::vector<uint32_t> part, tmp, ranges, vals;
     std::vector<float> v1, v2, v4, rgba;
-    if (!rd(f, part, 6) || !rd(f, tmp, n_active) || !rd(f, ranges, 2 * n_small) || !rd(f, vals, I) ||
+    if (!rd(f, part, 8) || !rd(f, tmp, n_active) || !rd(f, ranges, 2 * n_small) || !rd(f, vals, I) ||
         !rd(f, v1, 4 * N) || !rd(f, v2, 4 * N) || !rd(f, v4, 4 * N) || !rd(f, rgba, 4 * N)) {
         std::fprintf(stderr, "short read\n");
         return 2;
     }
     std::fclose(f);
     int64_t pairs = 0, kept = 0, reach = 0, bad = 0;
-    for (int type = 0; type < 3; ++type) {
-        const int step = type == 1 ? 2 : 1;  // periphery tiles sample their large tile at stride 2
+    for (int type = 0; type < 4; ++type) {
+        // periphery tiles sample their large tile at stride 2 (8x8 samples),
+        // quarter tiles at stride 4 (4x4 samples)
+        const int step = type == 3 ? 4 : (type == 1 ? 2 : 1);
+        const int side = type == 3 ? 4 : 8;
         for (uint32_t g = part[2 * type]; g < part[2 * type + 1]; ++g) {
             const uint32_t tg = tmp[g], lt = tg / 4, st = tg % 4;
             const int bx = (int)(lt % gw) * 16 + (int)(st % 2) * 8;
             const int by = (int)(lt / gw) * 16 + (int)(st / 2) * 8;
-            const float x0 = (float)bx, x1 = (float)(bx + 7 * step);
-            const float y0 = (float)by, y1 = (float)(by + 7 * step);
+            const float x0 = (float)bx, x1 = (float)(bx + (side - 1) * step);
+            const float y0 = (float)by, y1 = (float)(by + (side - 1) * step);
             for (uint32_t j = ranges[2 * (size_t)tg]; j < ranges[2 * (size_t)tg + 1]; ++j) {
                 const uint32_t p = vals[j];
                 const float *a1 = &v1[4 * (size_t)p], *a2 = &v2[4 * (size_t)p], *a4 = &v4[4 * (size_t)p];
                 const float o = rgba[4 * (size_t)p + 3];
                 const bool keep = fh_tile_may_touch(a1, a2, a4, o, x0, x1, y0, y1);
                 bool any = false;
-                for (int q = 0; q < 64 && !any; ++q) {
-                    const int px = bx + (q & 7) * step, py = by + (q >> 3) * step;
+                for (int q = 0; q < side * side && !any; ++q) {
+                    const int px = bx + (q % side) * step, py = by + (q / side) * step;
                     if (px < W && py < H) any = reaches(a1, a2, a4, o, (float)px, (float)py);
                 }
                 ++pairs;

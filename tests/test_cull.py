@@ -21,6 +21,8 @@ CASES = [
     Case("cull_scaled", 20000, 320, 240, seed=54, scale_modifier=3.0),
     Case("cull_aaa", 20000, 320, 240, seed=55, aaa=True),
     Case("cull_nofov", 20000, 320, 240, seed=56, foveation_radius_px=-1),
+    Case("cull_quarter", 20000, 480, 352, seed=57, foveation_radius_px=100, blend_width_px=40,
+         quarter_radius_px=140),
 ]
 
 

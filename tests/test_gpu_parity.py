@@ -42,13 +42,24 @@ CASES = [
     Case("aaa_fov", 30000, 640, 480, seed=41, aaa=True),
     Case("aaa_blur_chw", 30000, 480, 352, seed=42, aaa=True, blur=True, to_chw=True),
     Case("aaa_nofov_k16", 20000, 320, 240, seed=43, aaa=True, K=16, foveation_radius_px=-1),
+    # quarter-resolution level beyond quarter_radius_px
+    Case("quarter_640x480", 40000, 640, 480, seed=60, foveation_radius_px=150, blend_width_px=60,
+         quarter_radius_px=250),
+    Case("quarter_invisible_chw", 30000, 640, 480, seed=61, foveation_radius_px=120, blend_width_px=50,
+         quarter_radius_px=160, invisible_frac=0.3, to_chw=True),
+    Case("quarter_odd_size", 25000, 333, 257, seed=62, foveation_radius_px=64, blend_width_px=32,
+         quarter_radius_px=96),
+    Case("quarter_at_fovea_k1", 20000, 480, 352, seed=63, foveation_radius_px=100, blend_width_px=40,
+         quarter_radius_px=100, K=1),
+    Case("quarter_bg_tex_aaa", 20000, 480, 352, seed=64, foveation_radius_px=120, blend_width_px=50,
+         quarter_radius_px=200, background=2, aaa=True),
 ]
 
 
 def _gpu_render(case: Case, sc, cam, gaze, visible, bg, benchmark=False):
     from foveatedhtgscudabackend_amd import FoveatedHTGSRasterizer
     from foveatedhtgscudabackend_amd import scene as S
-    r = FoveatedHTGSRasterizer(case.foveation_radius_px, case.blend_width_px)
+    r = FoveatedHTGSRasterizer(case.foveation_radius_px, case.blend_width_px, case.quarter_radius_px)
     st = case.settings(cam, gaze, visible, bg)
     args = S.scene_to_torch(sc, "cuda")
     fn = r.benchmark if benchmark else r.render
@@ -75,7 +86,8 @@ def _compare_stages(case, o, h, view=0):
     np.testing.assert_array_equal(_stage(h, "tile_map", np.uint32, n_active, view), o["tile_map"])
     np.testing.assert_array_equal(_stage(h, "tile_type", np.uint8, n_active, view), o["tile_type"])
     np.testing.assert_array_equal(_stage(h, "tile_map_part", np.uint32, n_active, view), o["tile_map_part"])
-    np.testing.assert_array_equal(_stage(h, "partition", np.uint32, 6, view), o["partition"])
+    np.testing.assert_array_equal(_stage(h, "partition", np.uint32, 6, view), o["partition"][:6])
+    np.testing.assert_array_equal(_stage(h, "partition_quarter", np.uint32, 2, view), o["partition"][6:8])
     nt = _stage(h, "n_touched", np.uint32, N, view)
     np.testing.assert_array_equal(nt, o["n_touched"])
     vis = nt > 0
