@@ -6,9 +6,13 @@ one-view-per-launch path (two _C.render calls), on one MI355X.
 Workload (the stereo foveated configuration): 3M Gaussians from the seeded
 generator in a 20 m cube (log-scales N(-4.5,0.6), uniform quaternions,
 opacity sigmoid(N(0,1.5)), SH degree 3), two eyes of 2064x2208 at 100 deg
-horizontal fov with 63 mm IPD, K=8, reference foveation (375 px radius,
-125 px blend ring) around a seeded gaze within 15 deg of the axis, bilinear
-foveal upsampling. Synthetic data; inputs are resident in HBM before timing.
+horizontal fov with 63 mm IPD, K=8, around a seeded gaze within 15 deg of the
+axis, bilinear foveal upsampling. Three foveation levels (--levels 3, the
+default): full resolution within 15 deg of the gaze, half resolution to
+35 deg, quarter beyond -- radii fx*tan(angle) in pixels (232 and 606 px at
+this fov), with a blend ring of a third of the full-resolution radius as in
+the reference's 375/125 px; --levels 2 uses the reference's two levels
+(375/125 px). Synthetic data; inputs are resident in HBM before timing.
 
 Prints one JSON line. `roofline` is the dominant kernel's algorithmic rate
 from HIP events on the render stream; `cpu_baseline` is the scalar oracle on
@@ -30,10 +34,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
-# HBM bytes per blend dispatch (one eye) of the default workload, from the two
-# calibrated rocprofv3 PMC runs in profiles/stereo_3M_2064x2208_pmc_traffic.md
-# (FETCH_SIZE 285.2 MB + WRITE_SIZE 31.1 MB); reported only for that workload.
-BLEND_TRAFFIC_BYTES_DEFAULT = 316.3e6
+# HBM bytes per blend dispatch (one eye) of the stereo workload at its default
+# size, per foveation level count, from the two calibrated rocprofv3 PMC runs
+# in profiles/stereo_3M_2064x2208_{3,2}level_pmc_traffic.md (FETCH_SIZE +
+# WRITE_SIZE: 332.7 + 15.3 MB, and 285.2 + 31.1 MB); reported only for those.
+BLEND_TRAFFIC_BYTES = {3: 348.0e6, 2: 316.3e6}
 
 
 def percentile(xs, q):
@@ -54,15 +59,27 @@ def stage_bytes(counts: list, n_gauss: int, pixels: int) -> dict:
     }
 
 
-def cpu_baseline(scene: dict, cam, gaze, tile_stride: int) -> dict:
+def foveation_radii(levels: int, width: int, fov_deg: float) -> tuple[int, int, int]:
+    """(foveation radius, blend width, quarter radius) in pixels."""
+    if levels == 2:
+        return 375, 125, 0
+    fx = 0.5 * width / math.tan(math.radians(0.5 * fov_deg))
+    fr = int(round(fx * math.tan(math.radians(15.0))))
+    qr = int(round(fx * math.tan(math.radians(35.0))))
+    return fr, int(round(fr / 3.0)), qr
+
+
+def cpu_baseline(scene: dict, cam, gaze, tile_stride: int, radii: tuple, K: int) -> dict:
     """One eye of the frame on the scalar oracle, blending every
     `tile_stride`-th active tile; the blend time is scaled by the stride."""
     import oracle
     from foveatedhtgscudabackend_amd import tables as T
+    fr, bw, qr = radii
     words = T.visibility_mask_np(cam.width, cam.height)
     tabs = (words, T.visibility_area_table_np(words, cam.width, cam.height),
-            T.fovea_area_table_np(T.fovea_radius_tiles_for(cam.width, cam.height, 375, 125)))
-    view = oracle.view_from_camera(cam, gaze, tabs, K=8, blend_tile_stride=tile_stride)
+            T.fovea_area_table_np(T.fovea_radius_tiles_for(cam.width, cam.height, fr, bw)))
+    view = oracle.view_from_camera(cam, gaze, tabs, K=K, blend_tile_stride=tile_stride, foveation_radius_px=fr,
+                                   blend_width_px=bw, quarter_radius_px=qr)
     t0 = time.perf_counter()
     o = oracle.run(scene, view, want=("counts",))
     wall = time.perf_counter() - t0
@@ -82,6 +99,8 @@ def main() -> int:
     ap.add_argument("--fov", type=float, default=100.0)
     ap.add_argument("--cube", type=float, default=20.0)
     ap.add_argument("--K", type=int, default=8)
+    ap.add_argument("--levels", type=int, choices=(2, 3), default=3,
+                    help="stereo: 3 = full/half/quarter resolution at 15/35 deg, 2 = the reference's 375/125 px")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--xcd-run", type=int, default=None,
                     help="blend tiles per XCD run (library default when omitted)")
@@ -121,9 +140,11 @@ def main() -> int:
     gaze = S.seeded_gaze(left, args.seed + 2000)
     gen_s = time.perf_counter() - t0
     scene_t = S.scene_to_torch(sc, dev)
-    sl = S.settings_for(left, gaze, device=dev, K=args.K)
-    sr = S.settings_for(right, gaze, device=dev, K=args.K)
-    rast = FoveatedHTGSRasterizer()
+    radii = foveation_radii(args.levels, args.width, args.fov)
+    fr, bw, qr = radii
+    sl = S.settings_for(left, gaze, device=dev, K=args.K, foveation_radius_px=fr, blend_width_px=bw)
+    sr = S.settings_for(right, gaze, device=dev, K=args.K, foveation_radius_px=fr, blend_width_px=bw)
+    rast = FoveatedHTGSRasterizer(fr, bw, qr)
     handle = _lib.handle_for(torch.cuda.current_device())
     handle.set_option(_lib.OPT_STAGE_TIMING, 1)
     if args.xcd_run is not None:
@@ -213,7 +234,7 @@ def main() -> int:
         default_workload = (args.gaussians, args.width, args.height, args.K, args.fov) == (3_000_000, 2064, 2208, 8, 100.0)
         roof = {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / FP32_PEAK_TFLOPS,
-                "traffic": BLEND_TRAFFIC_BYTES_DEFAULT if default_workload else None, "kernel": "blend_kernel",
+                "traffic": BLEND_TRAFFIC_BYTES[args.levels] if default_workload else None, "kernel": "blend_kernel",
                 "note": "VALU-issue bound, FP32 without packed instructions (no MFMA); achieved = 28 FP32 ops x "
                         "64 x n_valid evaluations / blend time; traffic = HBM bytes per dispatch (one eye) "
                         "from the calibrated PMC runs"}
@@ -224,10 +245,11 @@ def main() -> int:
 
     cpu = None
     if args.cpu_baseline:
-        cb = cpu_baseline(sc, left, gaze, args.cpu_tile_stride)
+        cb = cpu_baseline(sc, left, gaze, args.cpu_tile_stride, radii, args.K)
         frame_ms = 2.0 * cb["eye_ms"]
         cpu = {"value": 1000.0 / frame_ms, "unit": "stereo frames/s", "cores": 1, "kind": "port",
-               "sample": (f"scalar oracle, left eye of the same frame (3M Gaussians, 2064x2208, K=8), "
+               "sample": (f"scalar oracle, left eye of the same frame ({args.gaussians} Gaussians, "
+                          f"{args.width}x{args.height}, K={args.K}, same foveation), "
                           f"every {args.cpu_tile_stride}th active tile blended and the blend time scaled by "
                           f"{args.cpu_tile_stride}; frame = 2 eyes; wall {cb['wall_s']:.1f} s"),
                "stage_ms": {k: round(v, 2) for k, v in cb["stage_ms"].items()}}
@@ -245,10 +267,14 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded generator, inputs resident in HBM)",
-        "config": {"workload": "stereo foveated frame: 3M Gaussians (20 m cube), 2 eyes x 2064x2208, "
-                               "100 deg fov, 63 mm IPD, K=8, foveation 375/125 px, shared preprocess",
+        "config": {"workload": (f"stereo foveated frame: {args.gaussians / 1e6:g}M Gaussians ({args.cube:g} m cube), "
+                                f"2 eyes x {args.width}x{args.height}, {args.fov:g} deg fov, 63 mm IPD, K={args.K}, "
+                                + (f"3 levels: full res < {fr} px (15 deg, blend ring {bw} px), half res < {qr} px "
+                                   f"(35 deg), quarter beyond" if qr else f"2 levels: foveation {fr}/{bw} px")
+                                + ", shared preprocess"),
                    "gaussians": args.gaussians, "width": args.width, "height": args.height, "K": args.K,
-                   "fov_deg": args.fov, "parallelism": "single GPU"},
+                   "fov_deg": args.fov, "foveation_radius_px": fr, "blend_width_px": bw,
+                   "quarter_radius_px": qr, "parallelism": "single GPU"},
         "p50_ms": percentile(lat, 50),
         "p99_ms": percentile(lat, 99),
         "plain_fps": plain_fps,

@@ -17,9 +17,9 @@ def _handle():
     return _lib, _lib.handle_for(0)
 
 
-def _render(args, st, fr=375, bw=125, blur=False, to_chw=False):
+def _render(args, st, fr=375, bw=125, blur=False, to_chw=False, qr=0):
     from foveatedhtgscudabackend_amd import FoveatedHTGSRasterizer
-    img = FoveatedHTGSRasterizer(fr, bw).render(*args, st, to_chw, blur, False)
+    img = FoveatedHTGSRasterizer(fr, bw, qr).render(*args, st, to_chw, blur, False)
     torch.cuda.synchronize()
     return img.cpu().numpy()
 
@@ -54,6 +54,45 @@ def test_frame_modes_agree_and_graph_replays_new_cameras(restore_mode):
         for rep in range(2):
             for k, st in enumerate(sts):
                 _same(_render(args, st, blur=True), ref[k], f"mode {mode} rep {rep} camera {k}")
+
+
+def test_quarter_level_frame_modes_agree_and_replay_new_gaze(restore_mode):
+    """The quarter partition moves with the gaze: a replayed graph (grids
+    sized by bounds) must match exact renders (grids from the read-back
+    partition) for every gaze, and switching the level off keeps the shape."""
+    from foveatedhtgscudabackend_amd import scene as S
+    lib, h = restore_mode
+    sc = S.make_scene(40000, seed=65, cube=10.0)
+    args = S.scene_to_torch(sc, "cuda")
+    cam = S.orbit_camera(800, 600, 90.0, seed=1065)
+    gazes = [np.array(g, np.float32) for g in ((400.0, 300.0), (30.0, 40.0), (790.0, 590.0))]
+    sts = [S.settings_for(cam, g, foveation_radius_px=120, blend_width_px=40) for g in gazes]
+    h.set_option(lib.OPT_FRAME_MODE, lib.FRAME_EXACT)
+    ref = [_render(args, st, 120, 40, qr=250) for st in sts]
+    two = _render(args, sts[0], 120, 40)
+    assert np.count_nonzero(ref[0] != two) > 1000, "the quarter level changes the periphery"
+    for mode in (lib.FRAME_DEVICE, lib.FRAME_GRAPH):
+        h.set_option(lib.OPT_FRAME_MODE, mode)
+        for rep in range(2):
+            for k, st in enumerate(sts):
+                _same(_render(args, st, 120, 40, qr=250), ref[k], f"mode {mode} rep {rep} gaze {k}")
+        _same(_render(args, sts[0], 120, 40), two, f"mode {mode}: quarter level off")
+
+
+def test_quarter_radius_validation():
+    from foveatedhtgscudabackend_amd import FoveatedHTGSRasterizer
+    from foveatedhtgscudabackend_amd import scene as S
+    sc = S.make_scene(1000, seed=66, cube=10.0)
+    args = S.scene_to_torch(sc, "cuda")
+    cam = S.orbit_camera(256, 192, 60.0, seed=1066)
+    st = S.settings_for(cam, np.array([128.0, 96.0], np.float32), foveation_radius_px=100, blend_width_px=30)
+    with pytest.raises(ValueError, match="quarter radius"):
+        FoveatedHTGSRasterizer(100, 30, 80).render(*args, st, False, False, False)
+    with pytest.raises(RuntimeError, match="blur_periphery"):
+        FoveatedHTGSRasterizer(100, 30, 150).render(*args, st, False, True, False)
+    st_off = S.settings_for(cam, np.array([128.0, 96.0], np.float32), foveation_radius_px=-1)
+    with pytest.raises(ValueError, match="quarter radius"):
+        FoveatedHTGSRasterizer(-1, 30, 150).render(*args, st_off, False, False, False)
 
 
 def test_device_frame_that_outgrows_its_buffers_is_rendered_exactly(restore_mode):
