@@ -35,6 +35,7 @@ OPT_SORT_DIGIT_BITS = 2
 OPT_STAGE_TIMING = 3
 OPT_BLEND_XCD_RUN = 4
 OPT_FRAME_MODE = 5
+OPT_VIEW_BRANCHES = 6
 FRAME_EXACT, FRAME_DEVICE, FRAME_GRAPH = 0, 1, 2
 NUM_STAGES = 11
 STAGE_NAMES = ("clear", "tiles", "preprocess", "scan", "sync", "instances", "sort", "ranges",

@@ -111,6 +111,7 @@ struct GraphEntry {
 };
 constexpr size_t kMaxGraphs = 64;
 constexpr int kMaxSlots = 4096;
+constexpr int kMaxBranches = 4;
 
 }  // namespace
 
@@ -127,6 +128,12 @@ struct fhtgs_context {
     int timing = 0;
     int frame_mode = kModeGraph;
     int last_mode = kModeExact;
+    // concurrent view branches: the frame's stream plus side streams (default
+    // priority), forked and joined with events
+    int branches = 2;
+    hipStream_t side[kMaxBranches - 1] = {};
+    hipEvent_t ev_fork = nullptr;
+    hipEvent_t ev_join[kMaxBranches - 1] = {};
     hipEvent_t ev[FHTGS_NUM_STAGES + 1] = {};
     float stage_ms[FHTGS_NUM_STAGES] = {};
     // per-frame parameters: ViewParams of every view, then SceneArgs of every
@@ -352,6 +359,7 @@ std::vector<int64_t> launch_signature(const fhtgs_context* h, const std::vector<
     g.push_back(n_views);
     g.push_back(h->digit_bits);
     g.push_back(h->xcd_run);
+    g.push_back(h->branches);
     g.push_back((int64_t)(uintptr_t)h->params.ptr);
     for (const PreGroup& pg : groups) {
         g.push_back(pg.scene);
@@ -379,11 +387,34 @@ std::vector<int64_t> launch_signature(const fhtgs_context* h, const std::vector<
     return g;
 }
 
+// Branch b of a frame on stream s: s itself for b = 0, else a side stream.
+hipStream_t branch_stream(fhtgs_context* h, int b, hipStream_t s) { return b == 0 ? s : h->side[b - 1]; }
+
+// Side streams 1..nb-1 wait for the work enqueued on s so far.
+int fork_branches(fhtgs_context* h, int nb, hipStream_t s) {
+    if (nb <= 1) return FHTGS_OK;
+    HCHK(hipEventRecord(h->ev_fork, s));
+    for (int b = 1; b < nb; ++b) HCHK(hipStreamWaitEvent(h->side[b - 1], h->ev_fork, 0));
+    return FHTGS_OK;
+}
+
+// s waits for the work enqueued on side streams 1..nb-1.
+int join_branches(fhtgs_context* h, int nb, hipStream_t s) {
+    for (int b = 1; b < nb; ++b) {
+        HCHK(hipEventRecord(h->ev_join[b - 1], h->side[b - 1]));
+        HCHK(hipStreamWaitEvent(s, h->ev_join[b - 1], 0));
+    }
+    return FHTGS_OK;
+}
+
 // Phase 1 of a frame: clears, tile classification, preprocess, scan.
 // dscenes[k]: device address of scene k's SceneArgs in the parameter block.
+// Untimed, with branches, the tile classifications (one workgroup each) run
+// on the side streams beside the preprocess; they are joined at the end.
 int enqueue_phase1(fhtgs_context* h, const std::vector<PreGroup>& groups, const fhtgs_scene* scenes,
                    const std::vector<const SceneArgs*>& dscenes, int n_views, bool timed, hipStream_t s) {
     int rc = 0;
+    const int nb = timed ? 1 : std::min(h->branches, n_views + 1);
     if (timed && (rc = record(h, 0, s))) return rc;
     for (int v = 0; v < n_views; ++v) {
         ViewSlot& sl = *h->cur[v];
@@ -393,7 +424,10 @@ int enqueue_phase1(fhtgs_context* h, const std::vector<PreGroup>& groups, const 
         HCHK(launch_clear_frame(sl.vp, sl.vb, s));
     }
     if (timed && (rc = record(h, 1, s))) return rc;
-    for (int v = 0; v < n_views; ++v) HCHK(launch_tiles((*h->cur[v]).vp, (*h->cur[v]).dvp, (*h->cur[v]).vb, s));
+    if ((rc = fork_branches(h, nb, s))) return rc;
+    for (int v = 0; v < n_views; ++v)
+        HCHK(launch_tiles((*h->cur[v]).vp, (*h->cur[v]).dvp, (*h->cur[v]).vb,
+                          branch_stream(h, nb > 1 ? 1 + v % (nb - 1) : 0, s)));
     if (timed && (rc = record(h, 2, s))) return rc;
     for (const PreGroup& pg : groups) {
         const ViewParams* hv[2] = {&(*h->cur[pg.v[0]]).vp, &(*h->cur[pg.v[pg.nv - 1]]).vp};
@@ -406,66 +440,99 @@ int enqueue_phase1(fhtgs_context* h, const std::vector<PreGroup>& groups, const 
         const int64_t N = (*h->cur[v]).n_prims;
         if (N > 0) HCHK(launch_scan_blocks((*h->cur[v]).vb, (uint32_t)((N + kPrimBlock - 1) / kPrimBlock), s));
     }
+    return join_branches(h, nb, s);
+}
+
+// Phase 2: instances, sort, ranges, output clear + blend, upsample.
+// exact: grids from the counts read back; otherwise from the capacities.
+int enqueue_instances(fhtgs_context* h, ViewSlot& slot, bool exact, hipStream_t s) {
+    const int64_t tiles = exact ? ((int64_t)slot.n_instances + kSortTile - 1) / kSortTile : slot.ib.tiles_cap;
+    HCHK(launch_sort_clear(slot.ib, slot.plan, tiles, s));
+    HCHK(launch_instances(slot.dvp, slot.vb, slot.ib, slot.plan, (uint32_t)slot.n_prims, slot.vp.n_large, s));
+    if (exact && h->keep_unsorted && slot.n_instances > 0) {
+        HCHK(launch_widen_keys(slot.ib.keys[0], slot.plan.key_bytes, slot.dbg_keys, slot.n_instances, s));
+        HCHK(hipMemcpyAsync(slot.dbg_vals, slot.ib.vals[0], (size_t)slot.n_instances * 4, hipMemcpyDeviceToDevice,
+                            s));
+    }
     return FHTGS_OK;
 }
 
-// Phase 2: instances, sort, ranges, output clear + blend, upsample; each
-// stage for every view before the next, so one event pair times a stage.
-// exact: grids from the counts read back; otherwise from the capacities.
+int enqueue_sort(fhtgs_context* h, ViewSlot& slot, bool exact, hipStream_t s) {
+    (void)h;
+    const int64_t tiles = exact ? ((int64_t)slot.n_instances + kSortTile - 1) / kSortTile : slot.ib.tiles_cap;
+    int src = 0;
+    for (int p = 0; p < slot.plan.passes; ++p) {
+        HCHK(launch_sort_pass(slot.ib, slot.plan, p, src, tiles, slot.vb.frame_info, 0u, s));
+        src = 1 - src;
+    }
+    slot.sorted_src = src;
+    return FHTGS_OK;
+}
+
+int enqueue_ranges(fhtgs_context* h, ViewSlot& slot, bool exact, hipStream_t s) {
+    (void)h;
+    HCHK(launch_ranges(slot.vb, slot.ib, slot.plan, slot.sorted_src,
+                       exact ? slot.n_instances : (uint32_t)slot.ib.capacity, s));
+    return FHTGS_OK;
+}
+
+int enqueue_blend(fhtgs_context* h, ViewSlot& slot, const fhtgs_view& view, bool exact, hipStream_t s) {
+    if (view.clear_output) HCHK(launch_clear_image(slot.vp, slot.dvp, slot.vb, s));
+    uint32_t grid = (uint32_t)slot.vp.n_small;
+    if (exact) {  // QUARTER tiles (the last partition) go four to a workgroup
+        const uint32_t nq = slot.host_fi[FI_PART + 7] - slot.host_fi[FI_PART + 6];
+        grid = slot.n_active - nq + (nq + 3u) / 4u;
+    }
+    HCHK(launch_blend(slot.dvp, slot.vb, slot.ib.vals[slot.sorted_src], view.K, grid, h->xcd_run, s));
+    return FHTGS_OK;
+}
+
+int enqueue_upsample(fhtgs_context* h, ViewSlot& slot, bool exact, hipStream_t s) {
+    (void)h;
+    uint32_t grid = (uint32_t)slot.vp.n_large;
+    if (exact) {
+        const uint32_t* part = slot.host_fi + FI_PART;
+        grid = (part[3] - part[2]) + (part[5] - part[4] + 3u) / 4u + (part[7] - part[6]);
+    }
+    HCHK(launch_upsample(slot.dvp, slot.vb, grid, s));
+    return FHTGS_OK;
+}
+
+// Timed: each stage for every view before the next, one event pair per
+// stage. Untimed: every view's chain on its branch (views round-robin over
+// the branches), so one view's sort overlaps another's blend.
 int enqueue_phase2(fhtgs_context* h, const fhtgs_view* views, int n_views, bool exact, bool timed,
                    hipStream_t s) {
     int rc = 0;
-    if (timed && (rc = record(h, 5, s))) return rc;
-    for (int v = 0; v < n_views; ++v) {
-        ViewSlot& slot = (*h->cur[v]);
-        const int64_t tiles = exact ? ((int64_t)slot.n_instances + kSortTile - 1) / kSortTile : slot.ib.tiles_cap;
-        HCHK(launch_sort_clear(slot.ib, slot.plan, tiles, s));
-        HCHK(launch_instances(slot.dvp, slot.vb, slot.ib, slot.plan, (uint32_t)slot.n_prims, s));
-        if (exact && h->keep_unsorted && slot.n_instances > 0) {
-            HCHK(launch_widen_keys(slot.ib.keys[0], slot.plan.key_bytes, slot.dbg_keys, slot.n_instances, s));
-            HCHK(hipMemcpyAsync(slot.dbg_vals, slot.ib.vals[0], (size_t)slot.n_instances * 4,
-                                hipMemcpyDeviceToDevice, s));
+    if (!timed) {
+        const int nb = std::min(h->branches, n_views);
+        if ((rc = fork_branches(h, nb, s))) return rc;
+        for (int v = 0; v < n_views; ++v) {
+            ViewSlot& slot = (*h->cur[v]);
+            hipStream_t st = branch_stream(h, v % nb, s);
+            if ((rc = enqueue_instances(h, slot, exact, st)) || (rc = enqueue_sort(h, slot, exact, st)) ||
+                (rc = enqueue_ranges(h, slot, exact, st)) || (rc = enqueue_blend(h, slot, views[v], exact, st)) ||
+                (rc = enqueue_upsample(h, slot, exact, st)))
+                return rc;
         }
+        return join_branches(h, nb, s);
     }
-    if (timed && (rc = record(h, 6, s))) return rc;
-    for (int v = 0; v < n_views; ++v) {
-        ViewSlot& slot = (*h->cur[v]);
-        const int64_t tiles = exact ? ((int64_t)slot.n_instances + kSortTile - 1) / kSortTile : slot.ib.tiles_cap;
-        int src = 0;
-        for (int p = 0; p < slot.plan.passes; ++p) {
-            HCHK(launch_sort_pass(slot.ib, slot.plan, p, src, tiles, slot.vb.frame_info, 0u, s));
-            src = 1 - src;
-        }
-        slot.sorted_src = src;
-    }
-    if (timed && (rc = record(h, 7, s))) return rc;
-    for (int v = 0; v < n_views; ++v) {
-        ViewSlot& slot = (*h->cur[v]);
-        HCHK(launch_ranges(slot.vb, slot.ib, slot.plan, slot.sorted_src,
-                           exact ? slot.n_instances : (uint32_t)slot.ib.capacity, s));
-    }
-    if (timed && (rc = record(h, 8, s))) return rc;
-    for (int v = 0; v < n_views; ++v) {
-        ViewSlot& slot = (*h->cur[v]);
-        if (views[v].clear_output) HCHK(launch_clear_image(slot.vp, slot.dvp, slot.vb, s));
-        uint32_t grid = (uint32_t)slot.vp.n_small;
-        if (exact) {  // QUARTER tiles (the last partition) go four to a workgroup
-            const uint32_t nq = slot.host_fi[FI_PART + 7] - slot.host_fi[FI_PART + 6];
-            grid = slot.n_active - nq + (nq + 3u) / 4u;
-        }
-        HCHK(launch_blend(slot.dvp, slot.vb, slot.ib.vals[slot.sorted_src], views[v].K, grid, h->xcd_run, s));
-    }
-    if (timed && (rc = record(h, 9, s))) return rc;
-    for (int v = 0; v < n_views; ++v) {
-        ViewSlot& slot = (*h->cur[v]);
-        uint32_t grid = (uint32_t)slot.vp.n_large;
-        if (exact) {
-            const uint32_t* part = slot.host_fi + FI_PART;
-            grid = (part[3] - part[2]) + (part[5] - part[4] + 3u) / 4u + (part[7] - part[6]);
-        }
-        HCHK(launch_upsample(slot.dvp, slot.vb, grid, s));
-    }
-    if (timed && (rc = record(h, 10, s))) return rc;
+    if ((rc = record(h, 5, s))) return rc;
+    for (int v = 0; v < n_views; ++v)
+        if ((rc = enqueue_instances(h, *h->cur[v], exact, s))) return rc;
+    if ((rc = record(h, 6, s))) return rc;
+    for (int v = 0; v < n_views; ++v)
+        if ((rc = enqueue_sort(h, *h->cur[v], exact, s))) return rc;
+    if ((rc = record(h, 7, s))) return rc;
+    for (int v = 0; v < n_views; ++v)
+        if ((rc = enqueue_ranges(h, *h->cur[v], exact, s))) return rc;
+    if ((rc = record(h, 8, s))) return rc;
+    for (int v = 0; v < n_views; ++v)
+        if ((rc = enqueue_blend(h, *h->cur[v], views[v], exact, s))) return rc;
+    if ((rc = record(h, 9, s))) return rc;
+    for (int v = 0; v < n_views; ++v)
+        if ((rc = enqueue_upsample(h, *h->cur[v], exact, s))) return rc;
+    if ((rc = record(h, 10, s))) return rc;
     return FHTGS_OK;
 }
 
@@ -729,8 +796,14 @@ int fhtgs_create(int device, fhtgs_handle* out) {
             return FHTGS_E_HIP;
         }
     }
-    if (hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking) != hipSuccess) {
-        delete h;
+    bool ok = hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) == hipSuccess;
+    for (int b = 0; ok && b < kMaxBranches - 1; ++b)
+        ok = hipStreamCreateWithFlags(&h->side[b], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&h->ev_join[b], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        (void)fhtgs_destroy(h);
+        (void)hipSetDevice(prev);
         return FHTGS_E_HIP;
     }
     (void)hipSetDevice(prev);
@@ -759,6 +832,11 @@ int fhtgs_destroy(fhtgs_handle h) {
     if (h->params_host) (void)hipHostFree(h->params_host);
     for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.exec);
     if (h->cap_stream) (void)hipStreamDestroy(h->cap_stream);
+    for (int b = 0; b < kMaxBranches - 1; ++b) {
+        if (h->side[b]) (void)hipStreamDestroy(h->side[b]);
+        if (h->ev_join[b]) (void)hipEventDestroy(h->ev_join[b]);
+    }
+    if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
     if (h->done) (void)hipEventDestroy(h->done);
     for (int i = 0; i <= FHTGS_NUM_STAGES; ++i)
         if (h->ev[i]) (void)hipEventDestroy(h->ev[i]);
@@ -791,6 +869,10 @@ int fhtgs_set_option(fhtgs_handle h, int32_t option, int64_t value) {
         case FHTGS_OPT_FRAME_MODE:
             if (value < 0 || value > 2) return set_err(h, FHTGS_E_INVALID, "frame mode must be 0, 1 or 2");
             h->frame_mode = (int)value;
+            return FHTGS_OK;
+        case FHTGS_OPT_VIEW_BRANCHES:
+            if (value < 1 || value > kMaxBranches) return set_err(h, FHTGS_E_INVALID, "view branches must be 1..4");
+            h->branches = (int)value;
             return FHTGS_OK;
         case FHTGS_OPT_BLEND_XCD_RUN:
             if (value < 0 || value > 4096) return set_err(h, FHTGS_E_INVALID, "XCD run must be 0..4096");
@@ -990,6 +1072,7 @@ int fhtgs_get_option(fhtgs_handle h, int32_t option, int64_t* value) {
         case FHTGS_OPT_STAGE_TIMING: *value = h->timing; return FHTGS_OK;
         case FHTGS_OPT_BLEND_XCD_RUN: *value = h->xcd_run; return FHTGS_OK;
         case FHTGS_OPT_FRAME_MODE: *value = h->frame_mode; return FHTGS_OK;
+        case FHTGS_OPT_VIEW_BRANCHES: *value = h->branches; return FHTGS_OK;
         default: return set_err(h, FHTGS_E_INVALID, "unknown option");
     }
 }

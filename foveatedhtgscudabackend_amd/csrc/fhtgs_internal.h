@@ -140,7 +140,7 @@ hipError_t launch_scan_blocks(const ViewBuffers& vb, uint32_t n_blocks, hipStrea
 hipError_t launch_sort_clear(const InstanceBuffers& ib, const SortPlan& plan, int64_t tiles,
                              hipStream_t s);
 hipError_t launch_instances(const ViewParams* dvp, const ViewBuffers& vb, const InstanceBuffers& ib,
-                            const SortPlan& plan, uint32_t n, hipStream_t s);
+                            const SortPlan& plan, uint32_t n, int n_large, hipStream_t s);
 hipError_t launch_sort_pass(const InstanceBuffers& ib, const SortPlan& plan, int pass, int src,
                             int64_t n_tiles, const uint32_t* frame_info, uint32_t n_explicit,
                             hipStream_t s);

@@ -260,6 +260,12 @@ int fhtgs_debug_counts(fhtgs_handle h, int32_t view, int64_t* counts);
  * A frame that needs more instances than its buffers hold in mode 1/2 is
  * rendered again in mode 0; the first frame of a new shape is always exact. */
 #define FHTGS_OPT_FRAME_MODE 5
+/* Concurrent view branches of an untimed frame: after the shared preprocess,
+ * each view's chain (instances, sort, ranges, blend, upsample) runs on one of
+ * this many streams of the handle (views round-robin), and tile
+ * classification overlaps the preprocess; 1 = one stream. Values 1..4,
+ * default 2. Images do not depend on it. */
+#define FHTGS_OPT_VIEW_BRANCHES 6
 int fhtgs_set_option(fhtgs_handle h, int32_t option, int64_t value);
 int fhtgs_get_option(fhtgs_handle h, int32_t option, int64_t* value);
 

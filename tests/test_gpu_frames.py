@@ -79,6 +79,36 @@ def test_quarter_level_frame_modes_agree_and_replay_new_gaze(restore_mode):
         _same(_render(args, sts[0], 120, 40), two, f"mode {mode}: quarter level off")
 
 
+def test_view_branches_leave_images_unchanged(restore_mode):
+    """Views on concurrent streams (and tile classification beside the
+    preprocess) render the same bits in every frame mode."""
+    from foveatedhtgscudabackend_amd import scene as S
+    from foveatedhtgscudabackend_amd.batch import BatchView, render_batch
+    lib, h = restore_mode
+    scenes = [S.scene_to_torch(S.make_scene(n, seed=70 + k, cube=10.0), "cuda") for k, n in enumerate((30000, 20000))]
+    views = []
+    for k in range(5):
+        cam = S.orbit_camera(320 + 32 * k, 240, 70.0, seed=1070 + k)
+        st = S.settings_for(cam, S.seeded_gaze(cam, 2070 + k), foveation_radius_px=100, blend_width_px=40)
+        views.append(BatchView(k % 2, st, foveation_radius_px=100, blend_width_px=40,
+                               quarter_radius_px=160 if k % 2 else 0))
+    h.set_option(lib.OPT_VIEW_BRANCHES, 1)
+    h.set_option(lib.OPT_FRAME_MODE, lib.FRAME_EXACT)
+    ref = [im.cpu().numpy() for im in render_batch(scenes, views)]
+    try:
+        for nb in (2, 3, 4):
+            h.set_option(lib.OPT_VIEW_BRANCHES, nb)
+            for mode in (lib.FRAME_EXACT, lib.FRAME_DEVICE, lib.FRAME_GRAPH):
+                h.set_option(lib.OPT_FRAME_MODE, mode)
+                for rep in range(2):
+                    got = render_batch(scenes, views)
+                    torch.cuda.synchronize()
+                    for k, (g, r) in enumerate(zip(got, ref)):
+                        _same(g.cpu().numpy(), r, f"branches {nb} mode {mode} rep {rep} view {k}")
+    finally:
+        h.set_option(lib.OPT_VIEW_BRANCHES, 2)
+
+
 def test_quarter_radius_validation():
     from foveatedhtgscudabackend_amd import FoveatedHTGSRasterizer
     from foveatedhtgscudabackend_amd import scene as S
