@@ -100,7 +100,7 @@ def main() -> int:
     ap.add_argument("--cube", type=float, default=20.0)
     ap.add_argument("--K", type=int, default=8)
     ap.add_argument("--levels", type=int, choices=(2, 3), default=3,
-                    help="stereo: 3 = full/half/quarter resolution at 15/35 deg, 2 = the reference's 375/125 px")
+                    help="foveation: 3 = full/half/quarter resolution at 15/35 deg, 2 = the reference's 375/125 px")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--xcd-run", type=int, default=None,
                     help="blend tiles per XCD run (library default when omitted)")
@@ -323,15 +323,17 @@ def bench_batched(args) -> int:
         h = int(rng.integers(480, 2209)) // 16 * 16
         sc = int(rng.integers(0, 4))
         fov = float(rng.uniform(60.0, 100.0))
-        fr = 375 if rng.random() < 0.5 else -1
+        foveated = rng.random() < 0.5
+        fr, bw, qr = foveation_radii(args.levels, w, fov) if foveated else (-1, 125, 0)
         if rng.random() < 0.5:
             cams = S.stereo_cameras(w, h, fov, seed=args.seed + 3000 + r)
         else:
             cams = (S.orbit_camera(w, h, fov, seed=args.seed + 3000 + r),)
         gaze = S.seeded_gaze(cams[0], args.seed + 5000 + r)
         for cam in cams:
-            st = S.settings_for(cam, gaze, K=args.K, foveation_radius_px=fr)
-            views.append(BatchView(sc, st, foveation_radius_px=fr, clear_output=False))
+            st = S.settings_for(cam, gaze, K=args.K, foveation_radius_px=fr, blend_width_px=bw)
+            views.append(BatchView(sc, st, foveation_radius_px=fr, blend_width_px=bw, quarter_radius_px=qr,
+                                   clear_output=False))
     handle = _lib.handle_for(torch.cuda.current_device())
     t0 = time.perf_counter()
     tuned = autotune(scenes, views, frames=2)
@@ -356,7 +358,8 @@ def bench_batched(args) -> int:
 
         def plain():
             for bv in views:
-                r = rast.setdefault(bv.foveation_radius_px, FoveatedHTGSRasterizer(bv.foveation_radius_px, 125))
+                key = (bv.foveation_radius_px, bv.blend_width_px, bv.quarter_radius_px)
+                r = rast.setdefault(key, FoveatedHTGSRasterizer(*key))
                 r.benchmark(*scenes[bv.scene], bv.settings, False, False, False)
         plain()
         torch.cuda.synchronize()
@@ -367,10 +370,11 @@ def bench_batched(args) -> int:
         plain_vps = max(3, args.steps // 4) * n_views / (time.perf_counter() - t1)
         handle.set_option(_lib.OPT_FRAME_MODE, _lib.FRAME_GRAPH)
     vps = args.steps * n_views / total
-    cfg = {"workload": "batched: 24 view requests (640x480..2064x2208, half stereo pairs, half foveated 375/125 px, "
-                       "K=8) over 4 resident scenes of 0.5/1.5/3/6 M Gaussians, one autotuned batch per step, "
-                       "graph replay", "views_per_step": n_views, "scenes_m": [n / 1e6 for n in sizes_m],
-           "parallelism": "single GPU"}
+    fov_desc = ("three levels at 15/35 deg" if args.levels == 3 else "375/125 px")
+    cfg = {"workload": f"batched: 24 view requests (640x480..2064x2208, 60..100 deg, half stereo pairs, half foveated "
+                       f"({fov_desc}), K={args.K}) over 4 resident scenes of 0.5/1.5/3/6 M Gaussians, one autotuned "
+                       "batch per step, graph replay", "views_per_step": n_views,
+           "scenes_m": [n / 1e6 for n in sizes_m], "levels": args.levels, "parallelism": "single GPU"}
     return _common(args, "batched views/sec, 1 GPU (vs one view per call)", vps, "views/s", 1e3 * total / args.steps,
                    cfg, {"p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99), "plain_views_per_s": plain_vps,
                          "batched_vs_plain": (vps / plain_vps) if plain_vps else None, "autotune": tuned,
@@ -388,9 +392,10 @@ def bench_serving(args) -> int:
     scenes = [S.scene_to_torch(S.make_scene(n, seed=args.seed + 20 + k, cube=args.cube), "cuda")
               for k, n in enumerate(sizes)]
     headsets = [V.HeadsetSim(h, h, seed=args.seed + 700 + h) for h in range(max(1, min(8, args.headsets)))]
-    reqs = V.make_requests(headsets, args.duration, args.width, args.height, args.fov)
+    radii = foveation_radii(args.levels, args.width, args.fov)
+    reqs = V.make_requests(headsets, args.duration, args.width, args.height, args.fov, foveation=radii)
     srv = V.Server(scenes, workers=args.serve_workers, max_batch_headsets=args.serve_batch,
-                   graph=bool(args.serve_graph))
+                   graph=bool(args.serve_graph), foveation=radii, keep_images=False)
     admitted = [hs.hid for hs in headsets if srv.admit(hs.hid, hs.scene, args.width, args.height)]
     reqs = [r for r in reqs if r.headset in srv.admitted]
     # warm-up: one frame per headset sizes every slot (exact) before the clock
@@ -403,8 +408,11 @@ def bench_serving(args) -> int:
     if not args.no_plain:
         for r in reqs:
             r.dropped, r.images = False, None
-        plain = V.run_plain(scenes, srv.admitted, reqs)
-    cfg = {"workload": f"serving: {len(headsets)} headsets x stereo 2064x2208 @ 90 Hz (jitter 1 ms), gaze random walk with "
+        plain = V.run_plain(scenes, srv.admitted, reqs, foveation=radii, keep_images=False)
+    fr, bw, qr = radii
+    lv = f"3 levels {fr}/{qr} px (15/35 deg), blend ring {bw} px" if qr else f"2 levels {fr}/{bw} px"
+    cfg = {"workload": f"serving: {len(headsets)} headsets x stereo {args.width}x{args.height} @ 90 Hz (jitter 1 ms), "
+                       f"{lv}, gaze random walk with "
                        f"saccades, 8 resident scenes 1..6 M ({sum(sizes) / 1e6:.0f} M Gaussians), "
                        f"{args.serve_workers} workers, batches of up to {args.serve_batch} headsets, latest frame wins",
            "duration_s": args.duration, "admitted": admitted, "graph": bool(args.serve_graph),

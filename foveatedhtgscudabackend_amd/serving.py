@@ -97,18 +97,24 @@ class HeadsetSim:
         return eye, self.gaze.copy()
 
 
+REFERENCE_FOVEATION = (_lib.DEFAULT_FOVEATION_RADIUS_PX, _lib.DEFAULT_BLEND_WIDTH_PX, 0)
+
+
 def stereo_views(eye: np.ndarray, gaze_deg: np.ndarray, width: int, height: int, fov_deg: float,
-                 ipd: float = 0.063, device="cuda") -> tuple:
+                 ipd: float = 0.063, device="cuda", foveation: tuple = REFERENCE_FOVEATION) -> tuple:
     """RasterizerSettings of both eyes for a head position looking at the
     origin, with the gaze projected into the left eye (as the stereo call
-    uses the left eye's gaze)."""
+    uses the left eye's gaze). ``foveation``: (radius, blend width, quarter
+    radius) in pixels; the culling tables follow the radius."""
     R = S.look_at_rotation(eye, np.zeros(3))
     left = S.make_camera(width, height, fov_deg, eye - 0.5 * ipd * R[0], R)
     right = S.make_camera(width, height, fov_deg, eye + 0.5 * ipd * R[0], R)
     gx = left.cx + left.fx * math.tan(math.radians(gaze_deg[0]))
     gy = left.cy + left.fy * math.tan(math.radians(gaze_deg[1]))
     gaze = np.array([np.clip(gx, 0, width - 1), np.clip(gy, 0, height - 1)], dtype=np.float32)
-    return S.settings_for(left, gaze, device=device), S.settings_for(right, gaze, device=device)
+    fr, bw = int(foveation[0]), int(foveation[1])
+    return (S.settings_for(left, gaze, device=device, foveation_radius_px=fr, blend_width_px=bw),
+            S.settings_for(right, gaze, device=device, foveation_radius_px=fr, blend_width_px=bw))
 
 
 # ---------------------------------------------------------------- requests
@@ -126,7 +132,7 @@ class Request:
 
 
 def make_requests(headsets: list, duration_s: float, width: int, height: int, fov_deg: float,
-                  device="cuda") -> list:
+                  device="cuda", foveation: tuple = REFERENCE_FOVEATION) -> list:
     """All requests of a run, in arrival order, with their cameras prepared
     (table building is caller-side work, done before the clock starts)."""
     reqs = []
@@ -135,7 +141,8 @@ def make_requests(headsets: list, duration_s: float, width: int, height: int, fo
         for k, t in enumerate(hs.arrivals(duration_s)):
             eye, gaze = hs.pose_at(t, t - last)
             last = t
-            reqs.append(Request(hs.hid, k, t, stereo_views(eye, gaze, width, height, fov_deg, device=device)))
+            reqs.append(Request(hs.hid, k, t, stereo_views(eye, gaze, width, height, fov_deg, device=device,
+                                                           foveation=foveation)))
     reqs.sort(key=lambda r: r.t_arrival)
     return reqs
 
@@ -155,9 +162,12 @@ class Server:
     """Continuous-batching renderer for admitted headsets (see module doc)."""
 
     def __init__(self, scenes: list, device: int = 0, workers: int = 2, max_batch_headsets: int = 8,
-                 graph: bool = True, reserve_bytes: int = 2 << 30):
+                 graph: bool = True, reserve_bytes: int = 2 << 30, foveation: tuple = REFERENCE_FOVEATION,
+                 keep_images: bool = True):
         self.scenes = scenes            # torch scene tuples, resident
         self.device = device
+        self.foveation = tuple(int(x) for x in foveation)  # (radius, blend width, quarter radius) px
+        self.keep_images = keep_images  # False: served images are dropped once rendered
         self.max_batch = max_batch_headsets
         self.reserve = reserve_bytes
         self.handles = [_lib.Handle(device) for _ in range(workers)]
@@ -221,9 +231,9 @@ class Server:
             for r in batch:
                 pair = []
                 for eye, st in enumerate(r.settings):
-                    v, img = _binding.view_struct(device, *st.as_tuple(), False, False, False,
-                                                  _lib.DEFAULT_FOVEATION_RADIUS_PX, _lib.DEFAULT_BLEND_WIDTH_PX,
-                                                  False, keep=keep)
+                    fr, bw, qr = self.foveation
+                    v, img = _binding.view_struct(device, *st.as_tuple(), False, False, False, fr, bw,
+                                                  False, keep=keep, quarter_radius_px=qr)
                     views.append(v)
                     vscene.append(sidx[self.admitted[r.headset]])
                     vslot.append(2 * r.headset + eye)
@@ -233,7 +243,7 @@ class Server:
             handle.render_batch(structs, views, vscene, stream.cuda_stream, vslot)
         end = time.perf_counter() - t0
         for r, im in zip(batch, images):
-            r.t_start, r.t_done, r.images = start, end, im
+            r.t_start, r.t_done, r.images = start, end, (im if self.keep_images else None)
         with self.lock:
             self.done.extend(batch)
 
@@ -277,14 +287,19 @@ def summarize(done: list, wall_s: float) -> dict:
     }
 
 
-def run_plain(scenes: list, admitted: dict, requests: list, device: int = 0) -> dict:
+def run_plain(scenes: list, admitted: dict, requests: list, device: int = 0,
+              foveation: tuple = REFERENCE_FOVEATION, keep_images: bool = True) -> dict:
     """The plain per-request path on the same arrivals: one worker, one
     stereo call per request (exact frames), requests served in arrival order
-    with the same latest-frame-wins rule."""
+    with the same latest-frame-wins rule. One untimed frame per headset
+    first sizes the buffers, as the server's warm-up does."""
     from .torch_bindings.rasterization import FoveatedHTGSRasterizer
     handle = _lib.handle_for(device)
     handle.set_option(_lib.OPT_FRAME_MODE, _lib.FRAME_EXACT)
-    rast = FoveatedHTGSRasterizer()
+    rast = FoveatedHTGSRasterizer(*foveation)
+    for r in {r.headset: r for r in reversed(requests)}.values():
+        rast.render_stereo(*scenes[admitted[r.headset]], *r.settings, False, False, False)
+    torch.cuda.synchronize(device)
     t0 = time.perf_counter()
     pending: dict = {}
     done: list = []
@@ -305,8 +320,9 @@ def run_plain(scenes: list, admitted: dict, requests: list, device: int = 0) -> 
         r = pending.pop(hid)
         r.t_start = time.perf_counter() - t0
         left, right = r.settings
-        r.images = rast.render_stereo(*scenes[admitted[hid]], left, right, False, False, False)
+        images = rast.render_stereo(*scenes[admitted[hid]], left, right, False, False, False)
         torch.cuda.synchronize(device)
+        r.images = images if keep_images else None
         r.t_done = time.perf_counter() - t0
         done.append(r)
     handle.set_option(_lib.OPT_FRAME_MODE, _lib.FRAME_GRAPH)

@@ -8,21 +8,22 @@ import torch
 pytestmark = pytest.mark.gpu
 
 
-def test_served_frames_equal_frames_rendered_alone():
+@pytest.mark.parametrize("foveation", [(375, 125, 0), (80, 27, 140)], ids=["two_levels", "three_levels"])
+def test_served_frames_equal_frames_rendered_alone(foveation):
     from foveatedhtgscudabackend_amd import FoveatedHTGSRasterizer
     from foveatedhtgscudabackend_amd import scene as S
     from foveatedhtgscudabackend_amd import serving as V
     scenes = [S.scene_to_torch(S.make_scene(n, seed=80 + k, cube=10.0), "cuda") for k, n in enumerate((20000, 35000))]
     headsets = [V.HeadsetSim(h, h % 2, seed=900 + h) for h in range(3)]
-    reqs = V.make_requests(headsets, 0.25, 320, 256, 100.0)
-    srv = V.Server(scenes, workers=2, max_batch_headsets=3)
+    reqs = V.make_requests(headsets, 0.25, 320, 256, 100.0, foveation=foveation)
+    srv = V.Server(scenes, workers=2, max_batch_headsets=3, foveation=foveation)
     for hs in headsets:
         assert srv.admit(hs.hid, hs.scene, 320, 256)
     stats = srv.run(reqs)
     print(stats)
     assert stats["requests"] == len(reqs)
     assert stats["served"] + stats["dropped"] == len(reqs) and stats["served"] > 0
-    rast = FoveatedHTGSRasterizer()
+    rast = FoveatedHTGSRasterizer(*foveation)
     checked = 0
     for r in srv.done:
         if r.dropped:
