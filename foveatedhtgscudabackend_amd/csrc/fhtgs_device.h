@@ -42,12 +42,14 @@ __device__ __forceinline__ v3 normalize3(v3 v) {
 // exhaustively by rcp_check, built next to the library); other x take the
 // IEEE division.
 __device__ __forceinline__ float rcp_rn(float x) {
+    const float r0 = __builtin_amdgcn_rcpf(x);
+    float r = fmaf(fmaf(-x, r0, 1.0f), r0, r0);
     const float ax = fabsf(x);
-    if (ax >= 0x1p-125f && ax <= 0x1p125f) {
-        const float r = __builtin_amdgcn_rcpf(x);
-        return fmaf(fmaf(-x, r, 1.0f), r, r);
-    }
-    return 1.0f / x;
+    const bool fast = ax >= 0x1p-125f && ax <= 0x1p125f;
+    // the IEEE division only when some lane needs it (a wave-uniform branch,
+    // so it is not evaluated speculatively)
+    if (__builtin_amdgcn_ballot_w64(!fast) != 0ull) r = fast ? r : 1.0f / x;
+    return r;
 }
 
 // Pointers read from the device parameter block are generic to the

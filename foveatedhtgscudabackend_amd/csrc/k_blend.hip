@@ -248,12 +248,12 @@ __global__ __launch_bounds__(64) void blend_kernel(const ViewParams* vpp, BlendA
 #pragma unroll
                     for (int c = 0; c < K; ++c) {
                         const bool lt = fd < cd[c];
-                        const float td = cd[c];
-                        const uint32_t tc = cc[c];
-                        cd[c] = lt ? fd : td;
-                        cc[c] = lt ? fc : tc;
-                        fd = lt ? td : fd;
-                        fc = lt ? tc : fc;
+                        const float nd = lt ? cd[c] : fd;  // carried on
+                        const uint32_t nc = lt ? cc[c] : fc;
+                        cd[c] = lt ? fd : cd[c];
+                        cc[c] = lt ? fc : cc[c];
+                        fd = nd;
+                        fc = nc;
                     }
                     const float s = 1.0f / 255.0f;
                     p0 = (float)(fc & 0xffu) * s;
