@@ -373,6 +373,11 @@ __global__ __launch_bounds__(kPrimBlock) void preprocess_kernel(PreprocessArgs g
 
     uint32_t touched[NV];
     bool visible[NV];
+    // per view: the record rows and bounds, stored after the SH loads (one
+    // counter tracks loads and stores, so loads after stores wait for them)
+    v4 R1[NV], R2[NV], R4[NV];
+    ushort4 bnd[NV];
+    float4 col[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) { touched[v] = 0u; visible[v] = false; }
     v3 mu{0.0f, 0.0f, 0.0f};
@@ -433,13 +438,10 @@ __global__ __launch_bounds__(kPrimBlock) void preprocess_kernel(PreprocessArgs g
                 if (nt == 0u) continue;
                 touched[v] = nt;
                 visible[v] = true;
-                const ViewBuffers& vb = a.vb[v];
-                vb.bounds[i] = make_ushort4((unsigned short)b0, (unsigned short)b1, (unsigned short)b2,
-                                            (unsigned short)b3);
-                float4* rec = vb.prim + 4 * (size_t)i;
-                rec[0] = st4(V1);
-                rec[1] = st4(V2);
-                rec[2] = st4(V4);
+                bnd[v] = make_ushort4((unsigned short)b0, (unsigned short)b1, (unsigned short)b2, (unsigned short)b3);
+                R1[v] = V1;
+                R2[v] = V2;
+                R4[v] = V4;
             }
         }
     }
@@ -492,7 +494,7 @@ __global__ __launch_bounds__(kPrimBlock) void preprocess_kernel(PreprocessArgs g
                 const ViewParams& vp = a.vp[v];
                 const v3 cam{vp.cam[0], vp.cam[1], vp.cam[2]};
                 const v3 rgb = sh_color(a.sh0 + 3 * (size_t)i, coef, mu, cam, vp.active_sh, vp.total_sh);
-                a.vb[v].prim[4 * (size_t)i + 3] = make_float4(rgb.x, rgb.y, rgb.z, opacity);
+                col[v] = make_float4(rgb.x, rgb.y, rgb.z, opacity);
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -507,8 +509,19 @@ __global__ __launch_bounds__(kPrimBlock) void preprocess_kernel(PreprocessArgs g
             const ViewParams& vp = a.vp[v];
             const v3 cam{vp.cam[0], vp.cam[1], vp.cam[2]};
             const v3 rgb = sh_color(a.sh0 + 3 * (size_t)i, a.shr, mu, cam, vp.active_sh, vp.total_sh);
-            a.vb[v].prim[4 * (size_t)i + 3] = make_float4(rgb.x, rgb.y, rgb.z, opacity);
+            col[v] = make_float4(rgb.x, rgb.y, rgb.z, opacity);
         }
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        if (!visible[v]) continue;
+        const ViewBuffers& vb = a.vb[v];
+        vb.bounds[i] = bnd[v];
+        float4* rec = vb.prim + 4 * (size_t)i;
+        rec[0] = st4(R1[v]);
+        rec[1] = st4(R2[v]);
+        rec[2] = st4(R4[v]);
+        rec[3] = col[v];
     }
 
 #pragma unroll
