@@ -410,7 +410,8 @@ int join_branches(fhtgs_context* h, int nb, hipStream_t s) {
 // Phase 1 of a frame: clears, tile classification, preprocess, scan.
 // dscenes[k]: device address of scene k's SceneArgs in the parameter block.
 // Untimed, with branches, the tile classifications (one workgroup each) run
-// on the side streams beside the preprocess; they are joined at the end.
+// on the side streams beside the preprocess, and the scans side by side
+// after it; they are joined at the end.
 int enqueue_phase1(fhtgs_context* h, const std::vector<PreGroup>& groups, const fhtgs_scene* scenes,
                    const std::vector<const SceneArgs*>& dscenes, int n_views, bool timed, hipStream_t s) {
     int rc = 0;
@@ -436,9 +437,13 @@ int enqueue_phase1(fhtgs_context* h, const std::vector<PreGroup>& groups, const 
         HCHK(launch_preprocess(hv, dv, vbs, pg.nv, dscenes[pg.scene], (uint32_t)scenes[pg.scene].n_primitives, s));
     }
     if (timed && (rc = record(h, 3, s))) return rc;
+    // the views' block-sum scans (one workgroup each) side by side
+    if ((rc = fork_branches(h, nb, s))) return rc;
     for (int v = 0; v < n_views; ++v) {
         const int64_t N = (*h->cur[v]).n_prims;
-        if (N > 0) HCHK(launch_scan_blocks((*h->cur[v]).vb, (uint32_t)((N + kPrimBlock - 1) / kPrimBlock), s));
+        if (N > 0)
+            HCHK(launch_scan_blocks((*h->cur[v]).vb, (uint32_t)((N + kPrimBlock - 1) / kPrimBlock),
+                                    branch_stream(h, v % nb, s)));
     }
     return join_branches(h, nb, s);
 }

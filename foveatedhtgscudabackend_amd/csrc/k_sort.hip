@@ -133,15 +133,17 @@ __global__ __launch_bounds__(kPrimBlock) void instances_kernel(const ViewParams*
         for (uint32_t k = 0; k < seq; ++k) {
             const uint32_t t = ty * gw + tx;
             if (mask_bit(s_mask, t)) {
-                const int reps = in_fovea(vp.fovea_r, (float)tx + 0.5f, (float)ty + 0.5f, gx, gy) ? kNumSub : 1;
-                for (int r = 0; r < reps; ++r, ++cur) {
-                    if (cur < end) {
-                        emit<KeyT, PASSES>(keys, vals, s_hist, plan, cur, t * kNumSub + (uint32_t)r, i);
-                        ++n_valid;
-                    } else {
-                        overflow = true;
-                    }
-                }
+                const uint32_t reps =
+                    in_fovea(vp.fovea_r, (float)tx + 0.5f, (float)ty + 0.5f, gx, gy) ? (uint32_t)kNumSub : 1u;
+                // slots past `end` are not written (the count under-covers: an error)
+                const uint32_t room = cur < end ? end - cur : 0u;
+                const uint32_t nw = min(reps, room);
+                overflow |= reps > room;
+#pragma unroll
+                for (uint32_t r = 0; r < (uint32_t)kNumSub; ++r)
+                    if (r < nw) emit<KeyT, PASSES>(keys, vals, s_hist, plan, cur + r, t * kNumSub + r, i);
+                n_valid += nw;
+                cur += reps;
             }
             if (++tx == bxe) { tx = bx; ++ty; }
         }
@@ -168,18 +170,16 @@ __global__ __launch_bounds__(kPrimBlock) void instances_kernel(const ViewParams*
                     if (vis) fov = in_fovea(vp.fovea_r, (float)cx + 0.5f, (float)cy + 0.5f, gx, gy);
                 }
                 const uint64_t wb = __ballot(vis), fb = __ballot(fov);
-                uint32_t off = c + (uint32_t)__popcll(wb & lt) + (kNumSub - 1) * (uint32_t)__popcll(fb & lt);
+                const uint32_t off = c + (uint32_t)__popcll(wb & lt) + (kNumSub - 1) * (uint32_t)__popcll(fb & lt);
                 if (vis) {
-                    const int reps = fov ? kNumSub : 1;
-                    for (int r = 0; r < reps; ++r, ++off) {
-                        if (off < send) {
-                            emit<KeyT, PASSES>(keys, vals, s_hist, plan, off, t * kNumSub + (uint32_t)r,
-                                               sprim);
-                            ++n_valid;
-                        } else {
-                            overflow = true;
-                        }
-                    }
+                    const uint32_t reps = fov ? (uint32_t)kNumSub : 1u;
+                    const uint32_t room = off < send ? send - off : 0u;
+                    const uint32_t nw = min(reps, room);
+                    overflow |= reps > room;
+#pragma unroll
+                    for (uint32_t r = 0; r < (uint32_t)kNumSub; ++r)
+                        if (r < nw) emit<KeyT, PASSES>(keys, vals, s_hist, plan, off + r, t * kNumSub + r, sprim);
+                    n_valid += nw;
                 }
                 c += (uint32_t)__popcll(wb) + (kNumSub - 1) * (uint32_t)__popcll(fb);
             }
