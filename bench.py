@@ -15,8 +15,10 @@ the reference's 375/125 px; --levels 2 uses the reference's two levels
 (375/125 px). Synthetic data; inputs are resident in HBM before timing.
 
 Prints one JSON line. `roofline` is the dominant kernel's algorithmic rate
-from HIP events on the render stream; `cpu_baseline` is the scalar oracle on
-the box's host (one thread) over a bounded sample of the same frame.
+from HIP events on the render stream, over frames in which each kernel runs
+alone (the stage profile; in the timed frames the two eyes' branches overlap);
+`cpu_baseline` is the scalar oracle on the box's host (one thread) over a
+bounded sample of the same frame.
 """
 from __future__ import annotations
 
@@ -37,8 +39,8 @@ FP32_PEAK_TFLOPS = 157.3
 # HBM bytes per blend dispatch (one eye) of the stereo workload at its default
 # size, per foveation level count, from the two calibrated rocprofv3 PMC runs
 # in profiles/stereo_3M_2064x2208_{3,2}level_pmc_traffic.md (FETCH_SIZE +
-# WRITE_SIZE: 332.7 + 15.3 MB, and 285.2 + 31.1 MB); reported only for those.
-BLEND_TRAFFIC_BYTES = {3: 348.0e6, 2: 316.3e6}
+# WRITE_SIZE: 339.6 + 15.4 MB, and 285.2 + 31.1 MB); reported only for those.
+BLEND_TRAFFIC_BYTES = {3: 355.0e6, 2: 316.3e6}
 
 
 def percentile(xs, q):
@@ -67,6 +69,25 @@ def foveation_radii(levels: int, width: int, fov_deg: float) -> tuple[int, int, 
     fr = int(round(fx * math.tan(math.radians(15.0))))
     qr = int(round(fx * math.tan(math.radians(35.0))))
     return fr, int(round(fr / 3.0)), qr
+
+
+def blend_evaluations(handle, view: int, n_small: int) -> int:
+    """Sum over the frame's partitioned tiles of instances x samples (64 for
+    FOVEA/BLENDED 8x8 tiles and PERIPHERY 16x16 tiles at stride 2, 16 for
+    QUARTER tiles at stride 4): the per-pixel evaluations the blend's
+    algorithm does."""
+    rng = np.frombuffer(handle.debug_read(view, "ranges", 8 * n_small), dtype=np.uint32).reshape(-1, 2)
+    part = np.frombuffer(handle.debug_read(view, "partition", 24), dtype=np.uint32)
+    quart = np.frombuffer(handle.debug_read(view, "partition_quarter", 8), dtype=np.uint32)
+    n_act = int(max(part.max(), quart.max()))
+    tmp = np.frombuffer(handle.debug_read(view, "tile_map_part", 4 * n_act), dtype=np.uint32)
+    inst = (rng[:, 1].astype(np.int64) - rng[:, 0])
+    evals = 0
+    for a, b, samples in ((part[0], part[1], 64), (part[2], part[3], 64), (part[4], part[5], 64),
+                          (quart[0], quart[1], 16)):
+        if b > a:
+            evals += samples * int(inst[tmp[a:b]].sum())
+    return evals
 
 
 def cpu_baseline(scene: dict, cam, gaze, tile_stride: int, radii: tuple, K: int) -> dict:
@@ -102,6 +123,8 @@ def main() -> int:
     ap.add_argument("--levels", type=int, choices=(2, 3), default=3,
                     help="foveation: 3 = full/half/quarter resolution at 15/35 deg, 2 = the reference's 375/125 px")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--view-branches", type=int, default=None,
+                    help="concurrent view branches of a frame (library default 2; 1 = views one after another)")
     ap.add_argument("--xcd-run", type=int, default=None,
                     help="blend tiles per XCD run (library default when omitted)")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-view-per-launch comparison")
@@ -115,7 +138,7 @@ def main() -> int:
                          "requests over 4 resident scenes; serving: 8 headsets at 90 Hz over 8 scenes")
     ap.add_argument("--duration", type=float, default=2.0, help="serving: seconds of requests")
     ap.add_argument("--serve-batch", type=int, default=1, help="serving: most headsets per batch")
-    ap.add_argument("--serve-workers", type=int, default=3, help="serving: batches in flight")
+    ap.add_argument("--serve-workers", type=int, default=2, help="serving: batches in flight")
     ap.add_argument("--headsets", type=int, default=8, help="serving: headsets (at most 8, one scene each)")
     ap.add_argument("--serve-graph", type=int, default=1, help="serving: replay captured graphs (1) or not (0)")
     args = ap.parse_args()
@@ -149,6 +172,8 @@ def main() -> int:
     handle.set_option(_lib.OPT_STAGE_TIMING, 1)
     if args.xcd_run is not None:
         handle.set_option(_lib.OPT_BLEND_XCD_RUN, args.xcd_run)
+    if args.view_branches is not None:
+        handle.set_option(_lib.OPT_VIEW_BRANCHES, args.view_branches)
     torch.cuda.synchronize()
 
     def stereo():
@@ -228,16 +253,18 @@ def main() -> int:
     dom = max(kernel_stages, key=lambda k: stage_ms[k])
     if dom == "blend":
         # VALU-bound: ~28 FP32 operations per (pixel, Gaussian) evaluation to
-        # the cutoff test; each valid instance is evaluated by 64 lanes
-        evals = sum(64 * max(c[5], 0) for c in counts_eye)
+        # the cutoff test, for every sample of every tile a Gaussian is listed in
+        evals = sum(blend_evaluations(handle, v, counts_eye[v][2]) for v in range(2))
         achieved = 28.0 * evals / (stage_ms["blend"] * 1e-3) / 1e12
         default_workload = (args.gaussians, args.width, args.height, args.K, args.fov) == (3_000_000, 2064, 2208, 8, 100.0)
         roof = {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / FP32_PEAK_TFLOPS,
                 "traffic": BLEND_TRAFFIC_BYTES[args.levels] if default_workload else None, "kernel": "blend_kernel",
+                "evaluations": evals,
                 "note": "VALU-issue bound, FP32 without packed instructions (no MFMA); achieved = 28 FP32 ops x "
-                        "64 x n_valid evaluations / blend time; traffic = HBM bytes per dispatch (one eye) "
-                        "from the calibrated PMC runs"}
+                        "sum over tiles of instances x samples / blend time of both eyes, each blend running "
+                        "alone (stage profile, HIP events); traffic = HBM bytes per dispatch (one eye) from the "
+                        "calibrated PMC runs"}
     else:
         ach = stage_gbs[dom] if stage_gbs.get(dom) else 0.0
         roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
