@@ -425,16 +425,33 @@ int enqueue_phase1(fhtgs_context* h, const std::vector<PreGroup>& groups, const 
         HCHK(launch_clear_frame(sl.vp, sl.vb, s));
     }
     if (timed && (rc = record(h, 1, s))) return rc;
-    if ((rc = fork_branches(h, nb, s))) return rc;
-    for (int v = 0; v < n_views; ++v)
-        HCHK(launch_tiles((*h->cur[v]).vp, (*h->cur[v]).dvp, (*h->cur[v]).vb,
-                          branch_stream(h, nb > 1 ? 1 + v % (nb - 1) : 0, s)));
-    if (timed && (rc = record(h, 2, s))) return rc;
-    for (const PreGroup& pg : groups) {
-        const ViewParams* hv[2] = {&(*h->cur[pg.v[0]]).vp, &(*h->cur[pg.v[pg.nv - 1]]).vp};
-        const ViewParams* dv[2] = {(*h->cur[pg.v[0]]).dvp, (*h->cur[pg.v[pg.nv - 1]]).dvp};
-        const ViewBuffers vbs[2] = {(*h->cur[pg.v[0]]).vb, (*h->cur[pg.v[pg.nv - 1]]).vb};
-        HCHK(launch_preprocess(hv, dv, vbs, pg.nv, dscenes[pg.scene], (uint32_t)scenes[pg.scene].n_primitives, s));
+    auto tiles = [&]() -> int {
+        for (int v = 0; v < n_views; ++v)
+            HCHK(launch_tiles((*h->cur[v]).vp, (*h->cur[v]).dvp, (*h->cur[v]).vb,
+                              branch_stream(h, nb > 1 ? 1 + v % (nb - 1) : 0, s)));
+        return FHTGS_OK;
+    };
+    auto preprocess = [&]() -> int {
+        for (const PreGroup& pg : groups) {
+            const ViewParams* hv[2] = {&(*h->cur[pg.v[0]]).vp, &(*h->cur[pg.v[pg.nv - 1]]).vp};
+            const ViewParams* dv[2] = {(*h->cur[pg.v[0]]).dvp, (*h->cur[pg.v[pg.nv - 1]]).dvp};
+            const ViewBuffers vbs[2] = {(*h->cur[pg.v[0]]).vb, (*h->cur[pg.v[pg.nv - 1]]).vb};
+            HCHK(launch_preprocess(hv, dv, vbs, pg.nv, dscenes[pg.scene], (uint32_t)scenes[pg.scene].n_primitives,
+                                   s));
+        }
+        return FHTGS_OK;
+    };
+    if (nb > 1) {
+        // the preprocess is enqueued (and, in a captured graph, ordered)
+        // before the tile kernels, which only wait for the clears
+        HCHK(hipEventRecord(h->ev_fork, s));
+        if ((rc = preprocess())) return rc;
+        for (int b = 1; b < nb; ++b) HCHK(hipStreamWaitEvent(h->side[b - 1], h->ev_fork, 0));
+        if ((rc = tiles())) return rc;
+    } else {
+        if ((rc = tiles())) return rc;
+        if (timed && (rc = record(h, 2, s))) return rc;
+        if ((rc = preprocess())) return rc;
     }
     if (timed && (rc = record(h, 3, s))) return rc;
     // the views' block-sum scans (one workgroup each) side by side
