@@ -422,7 +422,8 @@ def bench_serving(args) -> int:
     radii = foveation_radii(args.levels, args.width, args.fov)
     reqs = V.make_requests(headsets, args.duration, args.width, args.height, args.fov, foveation=radii)
     srv = V.Server(scenes, workers=args.serve_workers, max_batch_headsets=args.serve_batch,
-                   graph=bool(args.serve_graph), foveation=radii, keep_images=False)
+                   graph=bool(args.serve_graph), foveation=radii, keep_images=False,
+                   view_branches=args.view_branches)
     admitted = [hs.hid for hs in headsets if srv.admit(hs.hid, hs.scene, args.width, args.height)]
     reqs = [r for r in reqs if r.headset in srv.admitted]
     # warm-up: one frame per headset sizes every slot (exact) before the clock

@@ -163,7 +163,7 @@ class Server:
 
     def __init__(self, scenes: list, device: int = 0, workers: int = 2, max_batch_headsets: int = 8,
                  graph: bool = True, reserve_bytes: int = 2 << 30, foveation: tuple = REFERENCE_FOVEATION,
-                 keep_images: bool = True):
+                 keep_images: bool = True, view_branches: int | None = None):
         self.scenes = scenes            # torch scene tuples, resident
         self.device = device
         self.foveation = tuple(int(x) for x in foveation)  # (radius, blend width, quarter radius) px
@@ -173,6 +173,8 @@ class Server:
         self.handles = [_lib.Handle(device) for _ in range(workers)]
         for h in self.handles:
             h.set_option(_lib.OPT_FRAME_MODE, _lib.FRAME_GRAPH if graph else _lib.FRAME_EXACT)
+            if view_branches is not None:
+                h.set_option(_lib.OPT_VIEW_BRANCHES, int(view_branches))
         self.streams = [torch.cuda.Stream(device=device) for _ in range(workers)]
         self.admitted: dict = {}       # headset id -> scene index
         self.rejected: list = []
