@@ -28,7 +28,7 @@ DBG = {
     "tile_num": 1, "tile_map": 2, "tile_type": 3, "tile_map_part": 4, "partition": 6,
     "n_touched": 7, "bounds": 8, "vpmt1": 9, "vpmt2": 10, "vpmt4": 11, "rgba": 12,
     "keys_unsorted": 14, "vals_unsorted": 15, "keys_sorted": 16, "vals_sorted": 17,
-    "ranges": 18, "partition_quarter": 19,
+    "ranges": 18, "partition_quarter": 19, "blend_order": 20,
 }
 OPT_KEEP_UNSORTED = 1
 OPT_SORT_DIGIT_BITS = 2

@@ -226,6 +226,7 @@ void carve_fixed(void* base, int64_t N, int n_large, int n_small, ViewBuffers& v
     vb.tile_type = c.take<uint8_t>((size_t)n_small);
     vb.tile_map_part = c.take<uint32_t>((size_t)n_small);
     vb.ranges = c.take<uint2>((size_t)n_small);
+    vb.blend_order = c.take<uint32_t>((size_t)n_small);
     vb.n_touched = c.take<uint32_t>((size_t)N);
     vb.bounds = c.take<ushort4>((size_t)N);
     vb.prim = c.take<float4>((size_t)N * 4);
@@ -505,6 +506,7 @@ int enqueue_blend(fhtgs_context* h, ViewSlot& slot, const fhtgs_view& view, bool
         const uint32_t nq = slot.host_fi[FI_PART + 7] - slot.host_fi[FI_PART + 6];
         grid = slot.n_active - nq + (nq + 3u) / 4u;
     }
+    HCHK(launch_blend_order(slot.vb, slot.vp.n_small, s));
     HCHK(launch_blend(slot.dvp, slot.vb, slot.ib.vals[slot.sorted_src], view.K, grid, h->xcd_run, s));
     return FHTGS_OK;
 }
@@ -1213,6 +1215,12 @@ int fhtgs_debug_read(fhtgs_handle h, int32_t view, int32_t id, void* dst, int64_
         case FHTGS_DBG_TILE_MAP_PART: src = sl.vb.tile_map_part; bytes = na * 4; break;
         case FHTGS_DBG_PARTITION: src = sl.vb.frame_info + FI_PART; bytes = 24; break;
         case FHTGS_DBG_PARTITION_QUARTER: src = sl.vb.frame_info + FI_PART + 6; bytes = 8; break;
+        case FHTGS_DBG_BLEND_ORDER: {
+            const int64_t nq = (int64_t)sl.host_fi[FI_PART + 7] - sl.host_fi[FI_PART + 6];
+            src = sl.vb.blend_order;
+            bytes = (na - nq + (nq + 3) / 4) * 4;
+            break;
+        }
         case FHTGS_DBG_N_TOUCHED: src = sl.vb.n_touched; bytes = N * 4; break;
         case FHTGS_DBG_BOUNDS: src = sl.vb.bounds; bytes = N * 8; break;
         case FHTGS_DBG_VPMT1: src = sl.vb.prim + 0; bytes = N * 16; strided = true; break;

@@ -96,6 +96,7 @@ struct ViewBuffers {
     uint8_t* tile_type;       // n_small
     uint32_t* tile_map_part;  // n_small (partitioned by type)
     uint2* ranges;            // n_small
+    uint32_t* blend_order;    // n_small: blend work units, longest instance list first
     uint32_t* n_touched;      // N
     ushort4* bounds;          // N
     float4* prim;             // 4N: one 64-B record per primitive (VPMT1, VPMT2, VPMT4, rgba)
@@ -150,6 +151,7 @@ hipError_t launch_ranges(const ViewBuffers& vb, const InstanceBuffers& ib, const
                          int src, uint32_t n_grid, hipStream_t s);
 hipError_t launch_blend(const ViewParams* dvp, const ViewBuffers& vb, const uint32_t* sorted_vals,
                         int K, uint32_t grid, int xcd_run, hipStream_t s);
+hipError_t launch_blend_order(const ViewBuffers& vb, int n_small, hipStream_t s);
 hipError_t launch_upsample(const ViewParams* dvp, const ViewBuffers& vb, uint32_t grid, hipStream_t s);
 hipError_t launch_clear_image(const ViewParams& vp, const ViewParams* dvp, const ViewBuffers& vb,
                               hipStream_t s);

@@ -10,6 +10,8 @@
 // Extension: QUARTER tiles cover their 16x16 large tile at stride 4 (4x4
 // samples), four tiles per wave: 16-lane groups, each walking its own tile's
 // range in batches of 16 through its own quarter of the LDS batch.
+// Workgroups take their tiles from a work order with the longest instance
+// lists first (blend_order_kernel).
 #include "fhtgs_device.h"
 #include "fhtgs_cull.h"
 
@@ -114,6 +116,7 @@ __device__ __forceinline__ uint32_t pack_channel(float x) {
 
 struct BlendArgs {
     const uint32_t* frame_info;
+    const uint32_t* order;  // work units, longest first (blend_order_kernel)
     const uint32_t* tile_map_part;
     const uint2* ranges;
     const uint32_t* vals;
@@ -140,8 +143,11 @@ template <int K>
 __global__ __launch_bounds__(64) void blend_kernel(const ViewParams* vpp, BlendArgs a) {
     const ViewParams vp = load_view(vpp);
     __shared__ float4 sV1[64], sV2[64], sV4[64], sC[64];
-    const uint32_t g = xcd_remap(blockIdx.x, gridDim.x, a.xcd_run);
     const uint32_t* fi = a.frame_info;
+    const uint32_t g0 = xcd_remap(blockIdx.x, gridDim.x, a.xcd_run);
+    const uint32_t nq = fi[7] - fi[6];
+    if (g0 >= fi[FI_N_ACTIVE] - nq + (nq + 3u) / 4u) return;
+    const uint32_t g = a.order[g0];  // this workgroup's unit
     const uint32_t lane = threadIdx.x;
     int type;
     uint32_t j = g;  // this lane's partitioned tile
@@ -314,6 +320,75 @@ __global__ __launch_bounds__(64) void blend_kernel(const ViewParams* vpp, BlendA
     }
 }
 
+// Blend work order. A work unit is one active FOVEA, PERIPHERY or BLENDED
+// tile (partition index u) or four QUARTER tiles (unit u >= the non-QUARTER
+// count, in partition order). A unit's cost is its longest instance list;
+// units are bucketed by cost, two buckets per octave, and listed heaviest
+// bucket first, so that the longest workgroups start first and the short
+// ones fill the end of the launch instead of a few long ones trailing it.
+// The order inside a bucket is whatever the LDS atomics give: it decides
+// which workgroup renders a unit, never a pixel value.
+constexpr int kOrderBuckets = 48;
+constexpr int kOrderThreads = 1024;
+
+__device__ __forceinline__ uint32_t unit_cost(const uint32_t* tile_map_part, const uint2* ranges, uint32_t n_nq,
+                                              uint32_t na, uint32_t u) {
+    if (u < n_nq) {
+        const uint2 r = ranges[tile_map_part[u]];
+        return r.y - r.x;
+    }
+    const uint32_t j0 = n_nq + (u - n_nq) * 4u;
+    uint32_t c = 0u;
+    for (uint32_t q = 0; q < 4u; ++q) {
+        if (j0 + q < na) {
+            const uint2 r = ranges[tile_map_part[j0 + q]];
+            c = max(c, r.y - r.x);
+        }
+    }
+    return c;
+}
+
+// bucket 0 holds the longest lists: 2 floor(log2(c+1)) plus the octave half
+__device__ __forceinline__ uint32_t cost_bucket(uint32_t c) {
+    const uint32_t v = c + 1u;
+    const uint32_t l = 31u - (uint32_t)__clz(v);
+    const uint32_t half = l > 0u ? (v >> (l - 1u)) & 1u : 0u;
+    return (uint32_t)kOrderBuckets - 1u - min(2u * l + half, (uint32_t)kOrderBuckets - 1u);
+}
+
+__global__ __launch_bounds__(kOrderThreads) void blend_order_kernel(const uint32_t* frame_info,
+                                                                    const uint32_t* tile_map_part,
+                                                                    const uint2* ranges, uint32_t* order,
+                                                                    uint32_t n_small) {
+    __shared__ uint32_t s_cnt[kOrderBuckets];
+    const uint32_t na = min(frame_info[FI_N_ACTIVE], n_small);
+    const uint32_t nq = min(frame_info[FI_PART + 7] - frame_info[FI_PART + 6], na);
+    const uint32_t n_nq = na - nq;
+    const uint32_t units = n_nq + (nq + 3u) / 4u;
+    if (threadIdx.x < (uint32_t)kOrderBuckets) s_cnt[threadIdx.x] = 0u;
+    __syncthreads();
+    for (uint32_t u = threadIdx.x; u < units; u += kOrderThreads)
+        atomicAdd(&s_cnt[cost_bucket(unit_cost(tile_map_part, ranges, n_nq, na, u))], 1u);
+    __syncthreads();
+    if (threadIdx.x == 0) {  // exclusive scan: bucket starts
+        uint32_t run = 0u;
+        for (int b = 0; b < kOrderBuckets; ++b) {
+            const uint32_t c = s_cnt[b];
+            s_cnt[b] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    for (uint32_t u = threadIdx.x; u < units; u += kOrderThreads)
+        order[atomicAdd(&s_cnt[cost_bucket(unit_cost(tile_map_part, ranges, n_nq, na, u))], 1u)] = u;
+}
+
+hipError_t launch_blend_order(const ViewBuffers& vb, int n_small, hipStream_t s) {
+    blend_order_kernel<<<1, kOrderThreads, 0, s>>>(vb.frame_info, vb.tile_map_part, vb.ranges, vb.blend_order,
+                                                   (uint32_t)n_small);
+    return hipGetLastError();
+}
+
 // `grid` is the active tile count with QUARTER tiles counted four to a
 // workgroup, or n_small as a bound: workgroups outside the partition ranges
 // return at once.
@@ -323,6 +398,7 @@ hipError_t launch_blend(const ViewParams* dvp, const ViewBuffers& vb, const uint
     const uint32_t n_active = grid;
     BlendArgs a;
     a.frame_info = vb.frame_info;
+    a.order = vb.blend_order;
     a.tile_map_part = vb.tile_map_part;
     a.ranges = vb.ranges;
     a.vals = sorted_vals;

@@ -109,6 +109,47 @@ def test_view_branches_leave_images_unchanged(restore_mode):
         h.set_option(lib.OPT_VIEW_BRANCHES, 2)
 
 
+def _blend_units(h, n_small):
+    """(order, cost per unit) of the last frame's view 0: a unit is one
+    non-QUARTER tile or four consecutive QUARTER tiles; its cost is its
+    longest instance list."""
+    n_active = h.counts(0)[3]
+    quart = np.frombuffer(h.debug_read(0, "partition_quarter", 8), dtype=np.uint32).astype(np.int64)
+    nq = int(quart[1] - quart[0])
+    n_nq = n_active - nq
+    tmp = np.frombuffer(h.debug_read(0, "tile_map_part", 4 * n_active), dtype=np.uint32)
+    rng = np.frombuffer(h.debug_read(0, "ranges", 8 * n_small), dtype=np.uint32).reshape(-1, 2).astype(np.int64)
+    length = (rng[:, 1] - rng[:, 0])[tmp]
+    cost = list(length[:n_nq]) + [int(length[n_nq + 4 * k:min(n_active, n_nq + 4 * k + 4)].max())
+                                  for k in range((nq + 3) // 4)]
+    order = np.frombuffer(h.debug_read(0, "blend_order", 4 * len(cost)), dtype=np.uint32)
+    return order, np.array(cost, np.int64), nq
+
+
+def test_blend_order_is_a_longest_first_permutation(restore_mode):
+    """Every blend work unit appears once in the work order, and no unit
+    comes after one with less than half its cost (cost buckets are half an
+    octave wide), in every frame mode."""
+    from foveatedhtgscudabackend_amd import scene as S
+    lib, h = restore_mode
+    sc = S.make_scene(60000, seed=66, cube=10.0)
+    args = S.scene_to_torch(sc, "cuda")
+    cam = S.orbit_camera(800, 600, 90.0, seed=1066)
+    st = S.settings_for(cam, np.array((300.0, 250.0), np.float32), foveation_radius_px=120, blend_width_px=40)
+    gw, gh = -(-800 // 16), -(-600 // 16)
+    for mode in (lib.FRAME_EXACT, lib.FRAME_DEVICE, lib.FRAME_GRAPH):
+        h.set_option(lib.OPT_FRAME_MODE, mode)
+        for qr in (250, 0):
+            for _ in range(2):
+                _render(args, st, 120, 40, qr=qr)
+            order, cost, nq = _blend_units(h, 4 * gw * gh)
+            assert (nq > 0) == (qr > 0)
+            np.testing.assert_array_equal(np.sort(order), np.arange(len(cost)), err_msg=f"mode {mode} qr {qr}")
+            v = cost[order] + 1
+            assert np.all(v[1:] < 2 * np.minimum.accumulate(v)[:-1]), f"mode {mode} qr {qr}: order not longest first"
+            assert cost.max() > 4 * max(1, cost.min()), "the case has uneven lists"
+
+
 def test_quarter_radius_validation():
     from foveatedhtgscudabackend_amd import FoveatedHTGSRasterizer
     from foveatedhtgscudabackend_amd import scene as S
