@@ -26,9 +26,12 @@ constexpr int kTileBlended = 2;
 constexpr int kTileQuarter = 3;
 constexpr int kQuarterStride = 4;
 
-// Sort geometry: 256 threads x 16 keys per onesweep tile.
+// Sort geometry: 256 threads x 24 keys per onesweep tile. With three
+// workgroups per CU (137 VGPRs) the 4.8 M keys of one eye of a 2064x2208
+// stereo frame are 785 tiles, about one tile per resident workgroup, and the
+// look-back chain is a third shorter than with 16 keys.
 constexpr int kSortThreads = 256;
-constexpr int kSortItems = 16;
+constexpr int kSortItems = 24;
 constexpr int kSortTile = kSortThreads * kSortItems;
 constexpr int kMaxPasses = 4;
 constexpr int kMaxBins = 2048;

@@ -380,7 +380,8 @@ __device__ uint32_t block_excl_scan_lds(uint32_t* a, int count, uint32_t* s_tmp)
 constexpr int kLookback = 16;
 
 // BITS-bit digits; the tile is staged through LDS twice (keys, then values)
-// so that the staging area is 16 KB and five workgroups fit a CU.
+// so that the staging area is one tile of words (24 KB) and the LDS of three
+// workgroups fits a CU with room to spare.
 template <typename KeyT, int BITS>
 __global__ __launch_bounds__(kSortThreads) void onesweep_kernel(PassArgs a) {
     constexpr int BINS = 1 << BITS;

@@ -200,7 +200,8 @@ def test_stereo_shared_preprocess_equals_two_mono(oracle_lib):
 
 
 @pytest.mark.parametrize("n,key_bytes,end_bit", [
-    (0, 2, 12), (1, 2, 12), (4095, 2, 15), (4096, 4, 17), (4097, 4, 17), (100_000, 2, 16),
+    (0, 2, 12), (1, 2, 12), (4095, 2, 15), (4096, 4, 17), (4097, 4, 17), (6143, 4, 17), (6144, 2, 16),
+    (6145, 4, 17), (12_289, 4, 22), (100_000, 2, 16),
     (1_000_003, 4, 17), (2_000_000, 4, 22), (300_000, 4, 31)])
 def test_sort_pairs_is_stable_lsd(n, key_bytes, end_bit):
     from foveatedhtgscudabackend_amd import _lib
