@@ -330,23 +330,11 @@ __global__ __launch_bounds__(64) void blend_kernel(const ViewParams* vpp, BlendA
 // which workgroup renders a unit, never a pixel value.
 constexpr int kOrderBuckets = 48;
 constexpr int kOrderThreads = 1024;
-
-__device__ __forceinline__ uint32_t unit_cost(const uint32_t* tile_map_part, const uint2* ranges, uint32_t n_nq,
-                                              uint32_t na, uint32_t u) {
-    if (u < n_nq) {
-        const uint2 r = ranges[tile_map_part[u]];
-        return r.y - r.x;
-    }
-    const uint32_t j0 = n_nq + (u - n_nq) * 4u;
-    uint32_t c = 0u;
-    for (uint32_t q = 0; q < 4u; ++q) {
-        if (j0 + q < na) {
-            const uint2 r = ranges[tile_map_part[j0 + q]];
-            c = max(c, r.y - r.x);
-        }
-    }
-    return c;
-}
+// units per thread per chunk: all their loads are issued before the first
+// is used, and with at most one chunk (12288 units) the buckets stay in
+// registers between the count and the placement
+constexpr int kOrderPer = 12;
+constexpr uint32_t kOrderChunk = (uint32_t)kOrderThreads * kOrderPer;
 
 // bucket 0 holds the longest lists: 2 floor(log2(c+1)) plus the octave half
 __device__ __forceinline__ uint32_t cost_bucket(uint32_t c) {
@@ -356,19 +344,51 @@ __device__ __forceinline__ uint32_t cost_bucket(uint32_t c) {
     return (uint32_t)kOrderBuckets - 1u - min(2u * l + half, (uint32_t)kOrderBuckets - 1u);
 }
 
+// Buckets of units c0 + q * kOrderThreads + threadIdx.x (kOrderBuckets past
+// the last unit). A non-QUARTER unit u is partitioned tile u; QUARTER unit u
+// covers partitioned tiles n_nq + 4 (u - n_nq) + k, k < 4, below na.
+__device__ __forceinline__ void chunk_buckets(const uint32_t* tile_map_part, const uint2* ranges, uint32_t n_nq,
+                                              uint32_t na, uint32_t units, uint32_t c0, uint32_t bk[kOrderPer]) {
+    uint32_t tg[kOrderPer][4];
+#pragma unroll
+    for (int q = 0; q < kOrderPer; ++q) {
+        const uint32_t u = c0 + (uint32_t)q * kOrderThreads + threadIdx.x;
+        const bool qu = u >= n_nq;
+        const uint32_t j0 = qu ? n_nq + (u - n_nq) * 4u : u;
+        const uint32_t cnt = u >= units ? 0u : (qu ? min(4u, na - j0) : 1u);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) tg[q][k] = (uint32_t)k < cnt ? tile_map_part[j0 + k] : ~0u;
+    }
+#pragma unroll
+    for (int q = 0; q < kOrderPer; ++q) {
+        uint32_t c = 0u;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {  // unconditional loads (tile 0 stands in for none)
+            const bool on = tg[q][k] != ~0u;
+            const uint2 r = ranges[on ? tg[q][k] : 0u];
+            c = on ? max(c, r.y - r.x) : c;
+        }
+        bk[q] = tg[q][0] != ~0u ? cost_bucket(c) : (uint32_t)kOrderBuckets;
+    }
+}
+
 __global__ __launch_bounds__(kOrderThreads) void blend_order_kernel(const uint32_t* frame_info,
                                                                     const uint32_t* tile_map_part,
                                                                     const uint2* ranges, uint32_t* order,
                                                                     uint32_t n_small) {
-    __shared__ uint32_t s_cnt[kOrderBuckets];
+    __shared__ uint32_t s_cnt[kOrderBuckets + 1];
     const uint32_t na = min(frame_info[FI_N_ACTIVE], n_small);
     const uint32_t nq = min(frame_info[FI_PART + 7] - frame_info[FI_PART + 6], na);
     const uint32_t n_nq = na - nq;
     const uint32_t units = n_nq + (nq + 3u) / 4u;
-    if (threadIdx.x < (uint32_t)kOrderBuckets) s_cnt[threadIdx.x] = 0u;
+    if (threadIdx.x <= (uint32_t)kOrderBuckets) s_cnt[threadIdx.x] = 0u;
     __syncthreads();
-    for (uint32_t u = threadIdx.x; u < units; u += kOrderThreads)
-        atomicAdd(&s_cnt[cost_bucket(unit_cost(tile_map_part, ranges, n_nq, na, u))], 1u);
+    uint32_t bk[kOrderPer];
+    for (uint32_t c0 = 0; c0 < units; c0 += kOrderChunk) {
+        chunk_buckets(tile_map_part, ranges, n_nq, na, units, c0, bk);
+#pragma unroll
+        for (int q = 0; q < kOrderPer; ++q) atomicAdd(&s_cnt[bk[q]], 1u);  // slot kOrderBuckets: no unit
+    }
     __syncthreads();
     if (threadIdx.x == 0) {  // exclusive scan: bucket starts
         uint32_t run = 0u;
@@ -379,8 +399,13 @@ __global__ __launch_bounds__(kOrderThreads) void blend_order_kernel(const uint32
         }
     }
     __syncthreads();
-    for (uint32_t u = threadIdx.x; u < units; u += kOrderThreads)
-        order[atomicAdd(&s_cnt[cost_bucket(unit_cost(tile_map_part, ranges, n_nq, na, u))], 1u)] = u;
+    for (uint32_t c0 = 0; c0 < units; c0 += kOrderChunk) {
+        if (units > kOrderChunk) chunk_buckets(tile_map_part, ranges, n_nq, na, units, c0, bk);
+#pragma unroll
+        for (int q = 0; q < kOrderPer; ++q)
+            if (bk[q] < (uint32_t)kOrderBuckets)
+                order[atomicAdd(&s_cnt[bk[q]], 1u)] = c0 + (uint32_t)q * kOrderThreads + threadIdx.x;
+    }
 }
 
 hipError_t launch_blend_order(const ViewBuffers& vb, int n_small, hipStream_t s) {
