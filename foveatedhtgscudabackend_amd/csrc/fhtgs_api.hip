@@ -227,6 +227,8 @@ void carve_fixed(void* base, int64_t N, int n_large, int n_small, ViewBuffers& v
     vb.tile_map_part = c.take<uint32_t>((size_t)n_small);
     vb.ranges = c.take<uint2>((size_t)n_small);
     vb.blend_order = c.take<uint32_t>((size_t)n_small);
+    vb.order_bucket = c.take<uint8_t>((size_t)n_small);
+    vb.order_counts = c.take<uint32_t>(128);
     vb.n_touched = c.take<uint32_t>((size_t)N);
     vb.bounds = c.take<ushort4>((size_t)N);
     vb.prim = c.take<float4>((size_t)N * 4);

@@ -100,6 +100,8 @@ struct ViewBuffers {
     uint32_t* tile_map_part;  // n_small (partitioned by type)
     uint2* ranges;            // n_small
     uint32_t* blend_order;    // n_small: blend work units, longest instance list first
+    uint8_t* order_bucket;    // n_small: cost bucket per work unit
+    uint32_t* order_counts;   // 2 * 64: bucket histogram, then placement cursors (zeroed per frame)
     uint32_t* n_touched;      // N
     ushort4* bounds;          // N
     float4* prim;             // 4N: one 64-B record per primitive (VPMT1, VPMT2, VPMT4, rgba)

@@ -326,15 +326,12 @@ __global__ __launch_bounds__(64) void blend_kernel(const ViewParams* vpp, BlendA
 // units are bucketed by cost, two buckets per octave, and listed heaviest
 // bucket first, so that the longest workgroups start first and the short
 // ones fill the end of the launch instead of a few long ones trailing it.
-// The order inside a bucket is whatever the LDS atomics give: it decides
-// which workgroup renders a unit, never a pixel value.
-constexpr int kOrderBuckets = 48;
-constexpr int kOrderThreads = 1024;
-// units per thread per chunk: all their loads are issued before the first
-// is used, and with at most one chunk (12288 units) the buckets stay in
-// registers between the count and the placement
-constexpr int kOrderPer = 12;
-constexpr uint32_t kOrderChunk = (uint32_t)kOrderThreads * kOrderPer;
+// Two passes over many workgroups (bucket histogram, then placement); the
+// order inside a bucket is whatever the atomics give: it decides which
+// workgroup renders a unit, never a pixel value.
+constexpr int kOrderBuckets = 48;  // at most 64: the counters hold 2 x 64 words
+constexpr int kOrderThreads = 256;
+constexpr uint32_t kOrderMaxBlocks = 512;
 
 // bucket 0 holds the longest lists: 2 floor(log2(c+1)) plus the octave half
 __device__ __forceinline__ uint32_t cost_bucket(uint32_t c) {
@@ -344,73 +341,89 @@ __device__ __forceinline__ uint32_t cost_bucket(uint32_t c) {
     return (uint32_t)kOrderBuckets - 1u - min(2u * l + half, (uint32_t)kOrderBuckets - 1u);
 }
 
-// Buckets of units c0 + q * kOrderThreads + threadIdx.x (kOrderBuckets past
-// the last unit). A non-QUARTER unit u is partitioned tile u; QUARTER unit u
-// covers partitioned tiles n_nq + 4 (u - n_nq) + k, k < 4, below na.
-__device__ __forceinline__ void chunk_buckets(const uint32_t* tile_map_part, const uint2* ranges, uint32_t n_nq,
-                                              uint32_t na, uint32_t units, uint32_t c0, uint32_t bk[kOrderPer]) {
-    uint32_t tg[kOrderPer][4];
+struct OrderUnits {
+    uint32_t na, n_nq, units;
+};
+
+__device__ __forceinline__ OrderUnits order_units(const uint32_t* frame_info, uint32_t n_small) {
+    OrderUnits o;
+    o.na = min(frame_info[FI_N_ACTIVE], n_small);
+    const uint32_t nq = min(frame_info[FI_PART + 7] - frame_info[FI_PART + 6], o.na);
+    o.n_nq = o.na - nq;
+    o.units = o.n_nq + (nq + 3u) / 4u;
+    return o;
+}
+
+// Pass 1: each unit's bucket, and the bucket histogram (LDS per workgroup,
+// then one global add per bucket). A non-QUARTER unit u is partitioned tile
+// u; QUARTER unit u covers partitioned tiles n_nq + 4 (u - n_nq) + k, k < 4.
+__global__ __launch_bounds__(kOrderThreads) void blend_order_count_kernel(const uint32_t* frame_info,
+                                                                          const uint32_t* tile_map_part,
+                                                                          const uint2* ranges, uint8_t* bucket,
+                                                                          uint32_t* counts, uint32_t n_small) {
+    __shared__ uint32_t s_cnt[kOrderBuckets];
+    const OrderUnits o = order_units(frame_info, n_small);
+    if (threadIdx.x < (uint32_t)kOrderBuckets) s_cnt[threadIdx.x] = 0u;
+    __syncthreads();
+    for (uint32_t u = blockIdx.x * kOrderThreads + threadIdx.x; u < o.units; u += gridDim.x * kOrderThreads) {
+        const bool qu = u >= o.n_nq;
+        const uint32_t j0 = qu ? o.n_nq + (u - o.n_nq) * 4u : u;
+        const uint32_t cnt = qu ? min(4u, o.na - j0) : 1u;
+        uint32_t tg[4];
 #pragma unroll
-    for (int q = 0; q < kOrderPer; ++q) {
-        const uint32_t u = c0 + (uint32_t)q * kOrderThreads + threadIdx.x;
-        const bool qu = u >= n_nq;
-        const uint32_t j0 = qu ? n_nq + (u - n_nq) * 4u : u;
-        const uint32_t cnt = u >= units ? 0u : (qu ? min(4u, na - j0) : 1u);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) tg[q][k] = (uint32_t)k < cnt ? tile_map_part[j0 + k] : ~0u;
-    }
-#pragma unroll
-    for (int q = 0; q < kOrderPer; ++q) {
+        for (int k = 0; k < 4; ++k) tg[k] = (uint32_t)k < cnt ? tile_map_part[j0 + k] : 0u;
         uint32_t c = 0u;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {  // unconditional loads (tile 0 stands in for none)
-            const bool on = tg[q][k] != ~0u;
-            const uint2 r = ranges[on ? tg[q][k] : 0u];
-            c = on ? max(c, r.y - r.x) : c;
+            const uint2 r = ranges[tg[k]];
+            c = (uint32_t)k < cnt ? max(c, r.y - r.x) : c;
         }
-        bk[q] = tg[q][0] != ~0u ? cost_bucket(c) : (uint32_t)kOrderBuckets;
+        const uint32_t b = cost_bucket(c);
+        bucket[u] = (uint8_t)b;
+        atomicAdd(&s_cnt[b], 1u);
     }
+    __syncthreads();
+    if (threadIdx.x < (uint32_t)kOrderBuckets && s_cnt[threadIdx.x] != 0u)
+        atomicAdd(&counts[threadIdx.x], s_cnt[threadIdx.x]);
 }
 
-__global__ __launch_bounds__(kOrderThreads) void blend_order_kernel(const uint32_t* frame_info,
-                                                                    const uint32_t* tile_map_part,
-                                                                    const uint2* ranges, uint32_t* order,
-                                                                    uint32_t n_small) {
-    __shared__ uint32_t s_cnt[kOrderBuckets + 1];
-    const uint32_t na = min(frame_info[FI_N_ACTIVE], n_small);
-    const uint32_t nq = min(frame_info[FI_PART + 7] - frame_info[FI_PART + 6], na);
-    const uint32_t n_nq = na - nq;
-    const uint32_t units = n_nq + (nq + 3u) / 4u;
-    if (threadIdx.x <= (uint32_t)kOrderBuckets) s_cnt[threadIdx.x] = 0u;
-    __syncthreads();
-    uint32_t bk[kOrderPer];
-    for (uint32_t c0 = 0; c0 < units; c0 += kOrderChunk) {
-        chunk_buckets(tile_map_part, ranges, n_nq, na, units, c0, bk);
-#pragma unroll
-        for (int q = 0; q < kOrderPer; ++q) atomicAdd(&s_cnt[bk[q]], 1u);  // slot kOrderBuckets: no unit
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {  // exclusive scan: bucket starts
+// Pass 2: bucket starts from the histogram (every workgroup scans the same
+// 48 counts), a range per (workgroup round, bucket) from the global cursors,
+// ranks inside it from LDS atomics.
+__global__ __launch_bounds__(kOrderThreads) void blend_order_place_kernel(const uint32_t* frame_info,
+                                                                          const uint8_t* bucket, uint32_t* counts,
+                                                                          uint32_t* order, uint32_t n_small) {
+    __shared__ uint32_t s_start[kOrderBuckets], s_cnt[kOrderBuckets], s_base[kOrderBuckets];
+    const OrderUnits o = order_units(frame_info, n_small);
+    if (threadIdx.x == 0) {
         uint32_t run = 0u;
         for (int b = 0; b < kOrderBuckets; ++b) {
-            const uint32_t c = s_cnt[b];
-            s_cnt[b] = run;
-            run += c;
+            s_start[b] = run;
+            run += counts[b];
         }
     }
-    __syncthreads();
-    for (uint32_t c0 = 0; c0 < units; c0 += kOrderChunk) {
-        if (units > kOrderChunk) chunk_buckets(tile_map_part, ranges, n_nq, na, units, c0, bk);
-#pragma unroll
-        for (int q = 0; q < kOrderPer; ++q)
-            if (bk[q] < (uint32_t)kOrderBuckets)
-                order[atomicAdd(&s_cnt[bk[q]], 1u)] = c0 + (uint32_t)q * kOrderThreads + threadIdx.x;
+    for (uint32_t u0 = blockIdx.x * kOrderThreads; u0 < o.units; u0 += gridDim.x * kOrderThreads) {
+        if (threadIdx.x < (uint32_t)kOrderBuckets) s_cnt[threadIdx.x] = 0u;
+        __syncthreads();
+        const uint32_t u = u0 + threadIdx.x;
+        const bool on = u < o.units;
+        const uint32_t b = on ? bucket[u] : 0u;
+        const uint32_t r = on ? atomicAdd(&s_cnt[b], 1u) : 0u;
+        __syncthreads();
+        if (threadIdx.x < (uint32_t)kOrderBuckets && s_cnt[threadIdx.x] != 0u)
+            s_base[threadIdx.x] = atomicAdd(&counts[64 + threadIdx.x], s_cnt[threadIdx.x]);
+        __syncthreads();
+        if (on) order[s_start[b] + s_base[b] + r] = u;
+        __syncthreads();
     }
 }
 
 hipError_t launch_blend_order(const ViewBuffers& vb, int n_small, hipStream_t s) {
-    blend_order_kernel<<<1, kOrderThreads, 0, s>>>(vb.frame_info, vb.tile_map_part, vb.ranges, vb.blend_order,
-                                                   (uint32_t)n_small);
+    const uint32_t blocks = std::min<uint32_t>(kOrderMaxBlocks, ((uint32_t)n_small + kOrderThreads - 1u) / kOrderThreads);
+    blend_order_count_kernel<<<blocks, kOrderThreads, 0, s>>>(vb.frame_info, vb.tile_map_part, vb.ranges,
+                                                               vb.order_bucket, vb.order_counts, (uint32_t)n_small);
+    blend_order_place_kernel<<<blocks, kOrderThreads, 0, s>>>(vb.frame_info, vb.order_bucket, vb.order_counts,
+                                                               vb.blend_order, (uint32_t)n_small);
     return hipGetLastError();
 }
 

@@ -5,13 +5,15 @@
 
 namespace fhtgs {
 
-// Zeroes the instance-range table (inference.cu:124) and the frame-info words.
+// Zeroes the instance-range table (inference.cu:124), the frame-info words
+// and the blend work order's bucket counters.
 __global__ __launch_bounds__(256) void clear_frame_kernel(uint2* ranges, uint32_t n_small,
-                                                          uint32_t* frame_info) {
+                                                          uint32_t* frame_info, uint32_t* order_counts) {
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_small; i += stride)
         ranges[i] = make_uint2(0u, 0u);
     if (blockIdx.x == 0 && threadIdx.x < FI_WORDS) frame_info[threadIdx.x] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x < 128u) order_counts[threadIdx.x] = 0u;
 }
 
 hipError_t launch_clear_frame(const ViewParams& vp, const ViewBuffers& vb, hipStream_t s) {
@@ -19,7 +21,7 @@ hipError_t launch_clear_frame(const ViewParams& vp, const ViewBuffers& vb, hipSt
     uint32_t blocks = (n + 255u) / 256u;
     if (blocks < 1u) blocks = 1u;
     if (blocks > 1024u) blocks = 1024u;
-    clear_frame_kernel<<<blocks, 256, 0, s>>>(vb.ranges, n, vb.frame_info);
+    clear_frame_kernel<<<blocks, 256, 0, s>>>(vb.ranges, n, vb.frame_info, vb.order_counts);
     return hipGetLastError();
 }
 
