@@ -414,9 +414,12 @@ int join_branches(fhtgs_context* h, int nb, hipStream_t s) {
 // dscenes[k]: device address of scene k's SceneArgs in the parameter block.
 // Untimed, with branches, the tile classifications (one workgroup each) run
 // on the side streams beside the preprocess, and the scans side by side
-// after it; they are joined at the end.
+// after it; they are joined at the end. sort_clear: the instance buffers are
+// already sized (device counts, graph), so their sort counters are cleared
+// here, after each view's tile classification, instead of in phase 2.
 int enqueue_phase1(fhtgs_context* h, const std::vector<PreGroup>& groups, const fhtgs_scene* scenes,
-                   const std::vector<const SceneArgs*>& dscenes, int n_views, bool timed, hipStream_t s) {
+                   const std::vector<const SceneArgs*>& dscenes, int n_views, bool timed, bool sort_clear,
+                   hipStream_t s) {
     int rc = 0;
     const int nb = timed ? 1 : std::min(h->branches, n_views + 1);
     if (timed && (rc = record(h, 0, s))) return rc;
@@ -429,9 +432,12 @@ int enqueue_phase1(fhtgs_context* h, const std::vector<PreGroup>& groups, const 
     }
     if (timed && (rc = record(h, 1, s))) return rc;
     auto tiles = [&]() -> int {
-        for (int v = 0; v < n_views; ++v)
-            HCHK(launch_tiles((*h->cur[v]).vp, (*h->cur[v]).dvp, (*h->cur[v]).vb,
-                              branch_stream(h, nb > 1 ? 1 + v % (nb - 1) : 0, s)));
+        for (int v = 0; v < n_views; ++v) {
+            ViewSlot& sl = *h->cur[v];
+            hipStream_t st = branch_stream(h, nb > 1 ? 1 + v % (nb - 1) : 0, s);
+            HCHK(launch_tiles(sl.vp, sl.dvp, sl.vb, st));
+            if (sort_clear) HCHK(launch_sort_clear(sl.ib, sl.plan, sl.ib.tiles_cap, st));
+        }
         return FHTGS_OK;
     };
     auto preprocess = [&]() -> int {
@@ -471,8 +477,9 @@ int enqueue_phase1(fhtgs_context* h, const std::vector<PreGroup>& groups, const 
 // Phase 2: instances, sort, ranges, output clear + blend, upsample.
 // exact: grids from the counts read back; otherwise from the capacities.
 int enqueue_instances(fhtgs_context* h, ViewSlot& slot, bool exact, hipStream_t s) {
-    const int64_t tiles = exact ? ((int64_t)slot.n_instances + kSortTile - 1) / kSortTile : slot.ib.tiles_cap;
-    HCHK(launch_sort_clear(slot.ib, slot.plan, tiles, s));
+    // (non-exact frames cleared the sort counters in phase 1)
+    if (exact)
+        HCHK(launch_sort_clear(slot.ib, slot.plan, ((int64_t)slot.n_instances + kSortTile - 1) / kSortTile, s));
     HCHK(launch_instances(slot.dvp, slot.vb, slot.ib, slot.plan, (uint32_t)slot.n_prims, slot.vp.n_large, s));
     if (exact && h->keep_unsorted && slot.n_instances > 0) {
         HCHK(launch_widen_keys(slot.ib.keys[0], slot.plan.key_bytes, slot.dbg_keys, slot.n_instances, s));
@@ -675,7 +682,7 @@ int render_impl(fhtgs_context* h, const fhtgs_scene* scenes, int n_scenes, const
     HCHK(hipMemcpyAsync(h->params.ptr, h->params_host, pbytes, hipMemcpyHostToDevice, s));
 
     if (mode == kModeExact) {
-        if ((rc = enqueue_phase1(h, groups, scenes, dscenes, n_views, timed, s))) return rc;
+        if ((rc = enqueue_phase1(h, groups, scenes, dscenes, n_views, timed, false, s))) return rc;
         for (int v = 0; v < n_views; ++v)
             HCHK(hipMemcpyAsync((*h->cur[v]).host_fi, (*h->cur[v]).vb.frame_info, FI_WORDS * sizeof(uint32_t),
                                 hipMemcpyDeviceToHost, s));
@@ -697,7 +704,7 @@ int render_impl(fhtgs_context* h, const fhtgs_scene* scenes, int n_scenes, const
         }
         if ((rc = enqueue_phase2(h, views, n_views, true, timed, s))) return rc;
     } else if (mode == kModeDevice) {
-        if ((rc = enqueue_phase1(h, groups, scenes, dscenes, n_views, timed, s))) return rc;
+        if ((rc = enqueue_phase1(h, groups, scenes, dscenes, n_views, timed, true, s))) return rc;
         if (timed && (rc = record(h, 4, s))) return rc;
         if ((rc = enqueue_phase2(h, views, n_views, false, timed, s))) return rc;
     } else {
@@ -715,7 +722,7 @@ int render_impl(fhtgs_context* h, const fhtgs_scene* scenes, int n_scenes, const
                 h->graphs.erase(lru);
             }
             HCHK(hipStreamBeginCapture(h->cap_stream, hipStreamCaptureModeThreadLocal));
-            int crc = enqueue_phase1(h, groups, scenes, dscenes, n_views, false, h->cap_stream);
+            int crc = enqueue_phase1(h, groups, scenes, dscenes, n_views, false, true, h->cap_stream);
             if (!crc) crc = enqueue_phase2(h, views, n_views, false, false, h->cap_stream);
             hipGraph_t graph = nullptr;
             const hipError_t ce = hipStreamEndCapture(h->cap_stream, &graph);

@@ -148,35 +148,64 @@ hipError_t launch_tiles(const ViewParams& vp, const ViewParams* dvp, const ViewB
 }
 
 // Exclusive scan of the per-block tiles-touched sums in place; the total is
-// the instance count I (inference.cu:226-236).
-__global__ __launch_bounds__(1024) void scan_blocks_kernel(uint32_t* sums, uint32_t nb,
-                                                           uint32_t* frame_info) {
-    __shared__ uint32_t s_w[16];
-    const uint32_t per = (nb + blockDim.x - 1) / blockDim.x;
-    const uint32_t b0 = threadIdx.x * per;
-    const uint32_t b1 = min(nb, b0 + per);
-    uint32_t local = 0;
-    for (uint32_t b = b0; b < b1; ++b) local += sums[b];
-    const uint32_t lane = lane_id(), wid = threadIdx.x >> 6;
-    const uint32_t incl = wave_incl_scan(local);
-    if (lane == 63) s_w[wid] = incl;
-    __syncthreads();
-    uint32_t pre = 0, tot = 0;
-    for (uint32_t w = 0; w < (blockDim.x >> 6); ++w) {
-        if (w < wid) pre += s_w[w];
-        tot += s_w[w];
+// the instance count I (inference.cu:226-236). Chunks of 8192 sums are
+// loaded coalesced into LDS (all eight loads of a thread in flight), each
+// thread scans eight consecutive sums there, and the chunk is stored back
+// coalesced; a running total carries from chunk to chunk.
+constexpr uint32_t kScanPer = 8;
+constexpr uint32_t kScanThreads = 1024;
+constexpr uint32_t kScanChunk = kScanPer * kScanThreads;
+
+__global__ __launch_bounds__(kScanThreads) void scan_blocks_kernel(uint32_t* sums, uint32_t nb,
+                                                                   uint32_t* frame_info) {
+    __shared__ uint32_t s_v[kScanChunk];
+    __shared__ uint32_t s_w[kScanThreads / 64];
+    const uint32_t tid = threadIdx.x, lane = lane_id(), wid = tid >> 6;
+    uint32_t carry = 0;
+    for (uint32_t c0 = 0; c0 < nb; c0 += kScanChunk) {
+        uint32_t x[kScanPer];
+#pragma unroll
+        for (uint32_t k = 0; k < kScanPer; ++k) {
+            const uint32_t b = c0 + k * kScanThreads + tid;
+            x[k] = b < nb ? sums[b] : 0u;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kScanPer; ++k) s_v[k * kScanThreads + tid] = x[k];
+        __syncthreads();
+        uint32_t local = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kScanPer; ++k) {
+            x[k] = s_v[tid * kScanPer + k];
+            local += x[k];
+        }
+        const uint32_t incl = wave_incl_scan(local);
+        if (lane == 63) s_w[wid] = incl;
+        __syncthreads();
+        uint32_t pre = 0, tot = 0;
+        for (uint32_t w = 0; w < kScanThreads / 64; ++w) {
+            if (w < wid) pre += s_w[w];
+            tot += s_w[w];
+        }
+        uint32_t run = carry + pre + incl - local;
+#pragma unroll
+        for (uint32_t k = 0; k < kScanPer; ++k) {
+            s_v[tid * kScanPer + k] = run;
+            run += x[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t k = 0; k < kScanPer; ++k) {
+            const uint32_t b = c0 + k * kScanThreads + tid;
+            if (b < nb) sums[b] = s_v[k * kScanThreads + tid];
+        }
+        carry += tot;
+        __syncthreads();  // s_v and s_w are reused by the next chunk
     }
-    uint32_t run = pre + incl - local;
-    for (uint32_t b = b0; b < b1; ++b) {
-        const uint32_t v = sums[b];
-        sums[b] = run;
-        run += v;
-    }
-    if (threadIdx.x == 0) frame_info[FI_INSTANCES] = tot;
+    if (tid == 0) frame_info[FI_INSTANCES] = carry;
 }
 
 hipError_t launch_scan_blocks(const ViewBuffers& vb, uint32_t n_blocks, hipStream_t s) {
-    scan_blocks_kernel<<<1, 1024, 0, s>>>(vb.block_sums, n_blocks, vb.frame_info);
+    scan_blocks_kernel<<<1, kScanThreads, 0, s>>>(vb.block_sums, n_blocks, vb.frame_info);
     return hipGetLastError();
 }
 
