@@ -426,11 +426,9 @@ def bench_serving(args) -> int:
                    view_branches=args.view_branches)
     admitted = [hs.hid for hs in headsets if srv.admit(hs.hid, hs.scene, args.width, args.height)]
     reqs = [r for r in reqs if r.headset in srv.admitted]
-    # warm-up: one frame per headset sizes every slot (exact) before the clock
-    warm = [V.Request(r.headset, -1, 0.0, r.settings) for r in
-            {r.headset: r for r in reversed(reqs)}.values()]
-    V.Server.run(srv, warm)
-    srv.done, srv.closed = [], False
+    # warm-up before the clock: per worker and headset, one exact frame sizes
+    # the slots and one more captures the graph
+    srv.warm_up(reqs)
     stats = srv.run(reqs)
     plain = None
     if not args.no_plain:

@@ -257,6 +257,21 @@ class Server:
                 return
             self._render(w, batch, t0)
 
+    def warm_up(self, requests: list, frames: int = 2) -> None:
+        """Renders each headset's request `frames` times on every worker,
+        untimed: the first frame sizes the headset's slots (an exact frame),
+        the second captures the worker's graph for it, so that no timed
+        request pays for either."""
+        first = {r.headset: r for r in reversed(requests)}
+        t0 = time.perf_counter()
+        for w in range(len(self.handles)):
+            for _ in range(frames):
+                for hid in sorted(first):
+                    r = first[hid]
+                    self._render(w, [Request(hid, -1, 0.0, r.settings)], t0)
+        torch.cuda.synchronize(self.device)
+        self.done = []
+
     def run(self, requests: list) -> dict:
         """Plays the requests at their arrival times and returns statistics."""
         for r in requests:
@@ -264,16 +279,18 @@ class Server:
                 raise ValueError(f"headset {r.headset} was not admitted")
         t0 = time.perf_counter()
         threads = [threading.Thread(target=self._worker, args=(w, t0), daemon=True) for w in range(len(self.handles))]
-        for th in threads:
-            th.start()
-        for r in requests:
-            delay = r.t_arrival - (time.perf_counter() - t0)
-            if delay > 0:
-                time.sleep(delay)
-            self.submit(r)
-        self.close()
-        for th in threads:
-            th.join()
+        try:
+            for th in threads:
+                th.start()
+            for r in requests:
+                delay = r.t_arrival - (time.perf_counter() - t0)
+                if delay > 0:
+                    time.sleep(delay)
+                self.submit(r)
+        finally:
+            self.close()
+            for th in threads:
+                th.join()
         torch.cuda.synchronize(self.device)
         return summarize(self.done, time.perf_counter() - t0)
 
