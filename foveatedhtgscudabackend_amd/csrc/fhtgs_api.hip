@@ -709,6 +709,9 @@ int render_impl(fhtgs_context* h, const fhtgs_scene* scenes, int n_scenes, const
         if ((rc = enqueue_phase2(h, views, n_views, false, timed, s))) return rc;
     } else {
         if (h->timing) HCHK(hipEventRecord(h->ev[0], s));
+        // phase 1 straight onto the stream (its first kernels start at once);
+        // the captured graph holds phase 2 and is launched while phase 1 runs
+        if ((rc = enqueue_phase1(h, groups, scenes, dscenes, n_views, false, true, s))) return rc;
         std::vector<int64_t> sig = launch_signature(h, groups, scenes, views, n_views);
         GraphEntry* ge = nullptr;
         for (GraphEntry& e : h->graphs)
@@ -722,8 +725,7 @@ int render_impl(fhtgs_context* h, const fhtgs_scene* scenes, int n_scenes, const
                 h->graphs.erase(lru);
             }
             HCHK(hipStreamBeginCapture(h->cap_stream, hipStreamCaptureModeThreadLocal));
-            int crc = enqueue_phase1(h, groups, scenes, dscenes, n_views, false, true, h->cap_stream);
-            if (!crc) crc = enqueue_phase2(h, views, n_views, false, false, h->cap_stream);
+            int crc = enqueue_phase2(h, views, n_views, false, false, h->cap_stream);
             hipGraph_t graph = nullptr;
             const hipError_t ce = hipStreamEndCapture(h->cap_stream, &graph);
             if (crc) {
