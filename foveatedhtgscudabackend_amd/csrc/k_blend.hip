@@ -11,7 +11,7 @@
 // samples), four tiles per wave: 16-lane groups, each walking its own tile's
 // range in batches of 16 through its own quarter of the LDS batch.
 // Workgroups take their tiles from a work order with the longest instance
-// lists first (blend_order_kernel).
+// lists first (blend_order_count_kernel, blend_order_place_kernel).
 #include "fhtgs_device.h"
 #include "fhtgs_cull.h"
 
@@ -116,7 +116,7 @@ __device__ __forceinline__ uint32_t pack_channel(float x) {
 
 struct BlendArgs {
     const uint32_t* frame_info;
-    const uint32_t* order;  // work units, longest first (blend_order_kernel)
+    const uint32_t* order;  // work units, longest first (blend_order_place_kernel)
     const uint32_t* tile_map_part;
     const uint2* ranges;
     const uint32_t* vals;

@@ -126,24 +126,29 @@ def _blend_units(h, n_small):
     return order, np.array(cost, np.int64), nq
 
 
-def test_blend_order_is_a_longest_first_permutation(restore_mode):
+@pytest.mark.parametrize("w,h,fr,bw,qrs,n", [(800, 600, 120, 40, (250, 0), 60000),
+                                             (4096, 2304, -1, 125, (0,), 20000)])
+def test_blend_order_is_a_longest_first_permutation(restore_mode, w, h, fr, bw, qrs, n):
     """Every blend work unit appears once in the work order, and no unit
     comes after one with less than half its cost (cost buckets are half an
-    octave wide), in every frame mode."""
+    octave wide), in every frame mode; the 4096x2304 frame without foveation
+    has 147456 units, more than one placement round of the order kernels."""
     from foveatedhtgscudabackend_amd import scene as S
-    lib, h = restore_mode
-    sc = S.make_scene(60000, seed=66, cube=10.0)
+    lib, hd = restore_mode
+    sc = S.make_scene(n, seed=66, cube=10.0)
     args = S.scene_to_torch(sc, "cuda")
-    cam = S.orbit_camera(800, 600, 90.0, seed=1066)
-    st = S.settings_for(cam, np.array((300.0, 250.0), np.float32), foveation_radius_px=120, blend_width_px=40)
-    gw, gh = -(-800 // 16), -(-600 // 16)
+    cam = S.orbit_camera(w, h, 90.0, seed=1066)
+    st = S.settings_for(cam, np.array((0.4 * w, 0.4 * h), np.float32), foveation_radius_px=fr, blend_width_px=bw)
+    gw, gh = -(-w // 16), -(-h // 16)
     for mode in (lib.FRAME_EXACT, lib.FRAME_DEVICE, lib.FRAME_GRAPH):
-        h.set_option(lib.OPT_FRAME_MODE, mode)
-        for qr in (250, 0):
+        hd.set_option(lib.OPT_FRAME_MODE, mode)
+        for qr in qrs:
             for _ in range(2):
-                _render(args, st, 120, 40, qr=qr)
-            order, cost, nq = _blend_units(h, 4 * gw * gh)
+                _render(args, st, fr, bw, qr=qr)
+            order, cost, nq = _blend_units(hd, 4 * gw * gh)
             assert (nq > 0) == (qr > 0)
+            if fr < 0:
+                assert len(cost) == 4 * gw * gh
             np.testing.assert_array_equal(np.sort(order), np.arange(len(cost)), err_msg=f"mode {mode} qr {qr}")
             v = cost[order] + 1
             assert np.all(v[1:] < 2 * np.minimum.accumulate(v)[:-1]), f"mode {mode} qr {qr}: order not longest first"
