@@ -23,9 +23,11 @@ def _dev_ptr(t: torch.Tensor, name: str, dtypes, device) -> tuple[int, torch.Ten
     if device is not None and t.device != device:
         raise ValueError(f"{name} is on {t.device}, expected {device}")
     t = t.contiguous()
-    if t.data_ptr() % 16:
+    p = t.data_ptr()
+    if p % 16:
         t = t.clone()  # vector loads need 16-byte aligned rows
-    return t.data_ptr(), t
+        p = t.data_ptr()
+    return p, t
 
 
 _F32 = (torch.float32,)
@@ -77,10 +79,9 @@ def view_struct(device, M, VPM, VPR_inv, cam_position, gaze_position, visibility
         keep.append(tc)
         setattr(v, field, p)
     # the reference reads the gaze through a host pointer (inference.cu:79-82)
-    g = gaze_position.detach().to("cpu", torch.float32).reshape(-1)
-    if g.numel() != 2:
+    if gaze_position.numel() != 2:
         raise ValueError("gaze_position must have 2 elements")
-    v.gaze_x, v.gaze_y = float(g[0]), float(g[1])
+    v.gaze_x, v.gaze_y = gaze_position.detach().to("cpu", torch.float32).reshape(-1).tolist()
     gw, gh = (width + 15) // 16, (height + 15) // 16
     # extension: both summed-area tables None -> built on the device inside
     # the frame (and a None mask means every tile is visible)

@@ -7,6 +7,7 @@ Loading fails loudly when it is missing: there is no fallback path.
 from __future__ import annotations
 
 import ctypes
+import functools
 import os
 import threading
 
@@ -144,6 +145,7 @@ def load() -> ctypes.CDLL:
         return lib
 
 
+@functools.lru_cache(maxsize=256)
 def fovea_radius_tiles(width: int, height: int, foveation_radius_px: int = DEFAULT_FOVEATION_RADIUS_PX,
                        blend_width_px: int = DEFAULT_BLEND_WIDTH_PX) -> tuple[int, int]:
     """(fovea, blend-ring) radii in 16-px tiles; the fovea table is built for the first."""
