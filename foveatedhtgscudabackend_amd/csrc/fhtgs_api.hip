@@ -451,8 +451,8 @@ int enqueue_phase1(fhtgs_context* h, const std::vector<PreGroup>& groups, const 
         return FHTGS_OK;
     };
     if (nb > 1) {
-        // the preprocess is enqueued (and, in a captured graph, ordered)
-        // before the tile kernels, which only wait for the clears
+        // the preprocess is enqueued before the tile kernels, which only
+        // wait for the clears
         HCHK(hipEventRecord(h->ev_fork, s));
         if ((rc = preprocess())) return rc;
         for (int b = 1; b < nb; ++b) HCHK(hipStreamWaitEvent(h->side[b - 1], h->ev_fork, 0));

@@ -261,7 +261,8 @@ int fhtgs_debug_counts(fhtgs_handle h, int32_t view, int64_t* counts);
  *      size the instance buffers to them; the reference's behaviour
  *   1  device counts: no mid-frame wait; kernels read the counts on the GPU
  *      and the buffers hold the last exact frame's count plus headroom
- *   2  as 1, replayed from a captured hipGraph while the launch shape (views,
+ *   2  as 1, with the part after the preprocess (instances to upsample)
+ *      replayed from a captured hipGraph while the launch shape (views,
  *      sizes, K, scenes' sizes, buffers) stays the same (default)
  * A frame that needs more instances than its buffers hold in mode 1/2 is
  * rendered again in mode 0; the first frame of a new shape is always exact. */
