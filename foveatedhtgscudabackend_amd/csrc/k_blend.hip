@@ -329,8 +329,10 @@ __global__ __launch_bounds__(64) void blend_kernel(const ViewParams* vpp, BlendA
 // Two passes over many workgroups (bucket histogram, then placement); the
 // order inside a bucket is whatever the atomics give: it decides which
 // workgroup renders a unit, never a pixel value.
-constexpr int kOrderBuckets = 48;  // at most 64: the counters hold 2 x 64 words
+constexpr int kOrderBuckets = 48;
 constexpr int kOrderThreads = 256;
+static_assert(kOrderBuckets <= 64, "ViewBuffers::order_counts holds 2 x 64 words");
+static_assert(kOrderBuckets <= kOrderThreads, "one thread per bucket clears and flushes the counts");
 constexpr uint32_t kOrderMaxBlocks = 512;
 
 // bucket 0 holds the longest lists: 2 floor(log2(c+1)) plus the octave half
