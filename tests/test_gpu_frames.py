@@ -252,3 +252,27 @@ def test_slots_and_autotune_leave_images_unchanged(restore_mode):
     finally:
         h.set_option(lib.OPT_SORT_DIGIT_BITS, bits0)
         h.set_option(lib.OPT_BLEND_XCD_RUN, 16)
+
+
+@pytest.mark.parametrize("n_visible", [1, 0])
+def test_single_or_no_active_tile_renders_nothing(restore_mode, n_visible):
+    """One visible tile outside the fovea (n_active == 1: the reference
+    writes no partition range, so nothing is rendered) or none at all: the
+    image stays zero in every frame mode, with Gaussians in front of the
+    camera."""
+    from foveatedhtgscudabackend_amd import scene as S
+    lib, h = restore_mode
+    sc = S.make_scene(5000, seed=67, cube=4.0)
+    args = S.scene_to_torch(sc, "cuda")
+    cam = S.orbit_camera(64, 64, 60.0, seed=1067)
+    vis = np.zeros((4, 4), bool)
+    if n_visible:
+        vis[3, 3] = True
+    st = S.settings_for(cam, np.array((0.0, 0.0), np.float32), foveation_radius_px=16, blend_width_px=8,
+                        visible=vis)
+    for mode in (lib.FRAME_EXACT, lib.FRAME_DEVICE, lib.FRAME_GRAPH):
+        h.set_option(lib.OPT_FRAME_MODE, mode)
+        for _ in range(2):
+            img = _render(args, st, 16, 8)
+            assert h.counts(0)[3] == n_visible
+            assert not img.any(), f"mode {mode}: pixels rendered"
