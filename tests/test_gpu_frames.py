@@ -276,3 +276,27 @@ def test_single_or_no_active_tile_renders_nothing(restore_mode, n_visible):
             img = _render(args, st, 16, 8)
             assert h.counts(0)[3] == n_visible
             assert not img.any(), f"mode {mode}: pixels rendered"
+
+
+def test_repeated_frames_are_bit_identical(restore_mode):
+    """The blend's work order inside a cost bucket and the sort's tile
+    tickets depend on timing; the image must not: 40 replays of a stereo
+    frame with the quarter level give the same bits as an exact render."""
+    from foveatedhtgscudabackend_amd import FoveatedHTGSRasterizer
+    from foveatedhtgscudabackend_amd import scene as S
+    lib, h = restore_mode
+    sc = S.make_scene(150000, seed=68, cube=10.0)
+    args = S.scene_to_torch(sc, "cuda")
+    left, right = S.stereo_cameras(960, 1024, 100.0, seed=1068)
+    gaze = S.seeded_gaze(left, 2068)
+    sl = S.settings_for(left, gaze, foveation_radius_px=110, blend_width_px=36)
+    sr = S.settings_for(right, gaze, foveation_radius_px=110, blend_width_px=36)
+    r = FoveatedHTGSRasterizer(110, 36, 280)
+    h.set_option(lib.OPT_FRAME_MODE, lib.FRAME_EXACT)
+    ref = [im.cpu().numpy() for im in r.render_stereo(*args, sl, sr, False, False, False)]
+    h.set_option(lib.OPT_FRAME_MODE, lib.FRAME_GRAPH)
+    for k in range(40):
+        got = r.render_stereo(*args, sl, sr, False, False, False)
+        torch.cuda.synchronize()
+        for eye, (g, e) in enumerate(zip(got, ref)):
+            _same(g.cpu().numpy(), e, f"replay {k} eye {eye}")
