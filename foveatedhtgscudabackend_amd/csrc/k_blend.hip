@@ -323,24 +323,25 @@ __global__ __launch_bounds__(64) void blend_kernel(const ViewParams* vpp, BlendA
 // Blend work order. A work unit is one active FOVEA, PERIPHERY or BLENDED
 // tile (partition index u) or four QUARTER tiles (unit u >= the non-QUARTER
 // count, in partition order). A unit's cost is its longest instance list;
-// units are bucketed by cost, two buckets per octave, and listed heaviest
+// units are bucketed by cost, four buckets per octave, and listed heaviest
 // bucket first, so that the longest workgroups start first and the short
 // ones fill the end of the launch instead of a few long ones trailing it.
 // Two passes over many workgroups (bucket histogram, then placement); the
 // order inside a bucket is whatever the atomics give: it decides which
 // workgroup renders a unit, never a pixel value.
-constexpr int kOrderBuckets = 48;
+constexpr int kOrderBuckets = 64;
 constexpr int kOrderThreads = 256;
 static_assert(kOrderBuckets <= 64, "ViewBuffers::order_counts holds 2 x 64 words");
 static_assert(kOrderBuckets <= kOrderThreads, "one thread per bucket clears and flushes the counts");
 constexpr uint32_t kOrderMaxBlocks = 512;
 
-// bucket 0 holds the longest lists: 2 floor(log2(c+1)) plus the octave half
+// bucket 0 holds the longest lists: 4 floor(log2(c+1)) plus the quarter of
+// the octave (the two bits below the leading one)
 __device__ __forceinline__ uint32_t cost_bucket(uint32_t c) {
     const uint32_t v = c + 1u;
     const uint32_t l = 31u - (uint32_t)__clz(v);
-    const uint32_t half = l > 0u ? (v >> (l - 1u)) & 1u : 0u;
-    return (uint32_t)kOrderBuckets - 1u - min(2u * l + half, (uint32_t)kOrderBuckets - 1u);
+    const uint32_t qtr = l > 1u ? (v >> (l - 2u)) & 3u : (l == 1u ? (v & 1u) << 1 : 0u);
+    return (uint32_t)kOrderBuckets - 1u - min(4u * l + qtr, (uint32_t)kOrderBuckets - 1u);
 }
 
 struct OrderUnits {

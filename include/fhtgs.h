@@ -235,7 +235,7 @@ int fhtgs_fovea_radius_tiles(int32_t width, int32_t height, int32_t foveation_ra
 /* u32[units]: the blend's work order. A unit is one active FOVEA, PERIPHERY or
  * BLENDED tile (its partition index) or a group of four QUARTER tiles (the
  * non-QUARTER count plus the group index); units come longest instance list
- * first, in cost buckets of half an octave. Scheduling only: no pixel
+ * first, in cost buckets of a quarter octave. Scheduling only: no pixel
  * depends on it. */
 #define FHTGS_DBG_BLEND_ORDER 20
 int fhtgs_debug_read(fhtgs_handle h, int32_t view, int32_t id, void* host_dst,

@@ -130,8 +130,8 @@ def _blend_units(h, n_small):
                                              (4096, 2304, -1, 125, (0,), 20000)])
 def test_blend_order_is_a_longest_first_permutation(restore_mode, w, h, fr, bw, qrs, n):
     """Every blend work unit appears once in the work order, and no unit
-    comes after one with less than half its cost (cost buckets are half an
-    octave wide), in every frame mode; the 4096x2304 frame without foveation
+    comes after one with less than half its cost (cost buckets are a quarter
+    of an octave wide), in every frame mode; the 4096x2304 frame without foveation
     has 147456 units, more than one placement round of the order kernels."""
     from foveatedhtgscudabackend_amd import scene as S
     lib, hd = restore_mode
