@@ -49,15 +49,13 @@ def scene_struct(positions, scales, rotations, opacities, sh_0, sh_rest, keep: l
         raise ValueError(f"sh_0 must be (N,1,3), got {tuple(sh_0.shape)}")
     if sh_rest.dim() != 3 or sh_rest.shape[0] != n or sh_rest.shape[2] != 3:
         raise ValueError(f"sh_rest must be (N,S,3), got {tuple(sh_rest.shape)}")
-    s = _lib.Scene()
+    kw = {"n_primitives": n, "total_sh_bases": int(sh_rest.shape[1])}
     for field, t in (("positions", positions), ("scales", scales), ("rotations", rotations),
                      ("opacities", opacities), ("sh_0", sh_0), ("sh_rest", sh_rest)):
         p, tc = _dev_ptr(t, field, _F32, device)
         keep.append(tc)
-        setattr(s, field, p)
-    s.n_primitives = n
-    s.total_sh_bases = int(sh_rest.shape[1])
-    return s, device
+        kw[field] = p
+    return _lib.Scene(**kw), device
 
 
 def view_struct(device, M, VPM, VPR_inv, cam_position, gaze_position, visibility_mask,
@@ -66,7 +64,7 @@ def view_struct(device, M, VPM, VPR_inv, cam_position, gaze_position, visibility
                 center_x, center_y, near_plane, far_plane, scale_modifier, to_chw,
                 blur_periphery, anti_aliasing, foveation_radius_px, blend_width_px,
                 clear_output, keep: list, quarter_radius_px: int = 0) -> tuple:
-    v = _lib.View()
+    kw: dict = {}  # View fields, set at once at the end (faster than one setattr each)
     width, height = int(width), int(height)
     if M.numel() < 12:
         raise ValueError("M must hold at least 3 rows of 4")
@@ -77,11 +75,11 @@ def view_struct(device, M, VPM, VPR_inv, cam_position, gaze_position, visibility
     for field, t in (("M", M), ("VPM", VPM), ("VPR_inv", VPR_inv), ("cam_position", cam_position)):
         p, tc = _dev_ptr(t, field, _F32, device)
         keep.append(tc)
-        setattr(v, field, p)
+        kw[field] = p
     # the reference reads the gaze through a host pointer (inference.cu:79-82)
     if gaze_position.numel() != 2:
         raise ValueError("gaze_position must have 2 elements")
-    v.gaze_x, v.gaze_y = gaze_position.detach().to("cpu", torch.float32).reshape(-1).tolist()
+    kw["gaze_x"], kw["gaze_y"] = gaze_position.detach().to("cpu", torch.float32).reshape(-1).tolist()
     gw, gh = (width + 15) // 16, (height + 15) // 16
     # extension: both summed-area tables None -> built on the device inside
     # the frame (and a None mask means every tile is visible)
@@ -102,45 +100,45 @@ def view_struct(device, M, VPM, VPR_inv, cam_position, gaze_position, visibility
                      ("visibility_mask_area_table", visibility_mask_area_table),
                      ("fovea_mask_area_table", fovea_mask_area_table)):
         if t is None:
-            setattr(v, field, None)
+            kw[field] = None
             continue
         p, tc = _dev_ptr(t, field, _U32, device)
         keep.append(tc)
-        setattr(v, field, p)
+        kw[field] = p
     bg_type = int(background_model_type)
     p, tc = _dev_ptr(background_model_data, "background_model_data", _F32, device)
     keep.append(tc)
-    v.background_model_data = p
-    v.background_model_type = bg_type
+    kw["background_model_data"] = p
+    kw["background_model_type"] = bg_type
     if bg_type in (2, 3):
         if background_model_data.dim() != 3 or background_model_data.shape[0] != 3:
             raise ValueError("texture background must be (3,h,w)")
-        v.background_width = int(background_model_data.shape[2])
-        v.background_height = int(background_model_data.shape[1])
+        kw["background_width"] = int(background_model_data.shape[2])
+        kw["background_height"] = int(background_model_data.shape[1])
     elif bg_type == 1 and background_model_data.numel() < 48:
         raise ValueError("SH background needs 16x3 coefficients")
-    v.K = int(K)
-    v.active_sh_bases = int(active_sh_bases)
-    v.width, v.height = width, height
-    v.focal_x, v.focal_y = float(focal_x), float(focal_y)
-    v.center_x, v.center_y = float(center_x), float(center_y)
-    v.near_plane, v.far_plane = float(near_plane), float(far_plane)
-    v.scale_modifier = float(scale_modifier)
-    v.to_chw = 1 if to_chw else 0
-    v.blur_periphery = 1 if blur_periphery else 0
-    v.anti_aliasing = 1 if anti_aliasing else 0
-    v.foveation_radius_px = int(foveation_radius_px)
-    v.blend_width_px = int(blend_width_px)
-    v.quarter_radius_px = int(quarter_radius_px)
-    v.clear_output = 1 if clear_output else 0
+    kw["K"] = int(K)
+    kw["active_sh_bases"] = int(active_sh_bases)
+    kw["width"], kw["height"] = width, height
+    kw["focal_x"], kw["focal_y"] = float(focal_x), float(focal_y)
+    kw["center_x"], kw["center_y"] = float(center_x), float(center_y)
+    kw["near_plane"], kw["far_plane"] = float(near_plane), float(far_plane)
+    kw["scale_modifier"] = float(scale_modifier)
+    kw["to_chw"] = 1 if to_chw else 0
+    kw["blur_periphery"] = 1 if blur_periphery else 0
+    kw["anti_aliasing"] = 1 if anti_aliasing else 0
+    kw["foveation_radius_px"] = int(foveation_radius_px)
+    kw["blend_width_px"] = int(blend_width_px)
+    kw["quarter_radius_px"] = int(quarter_radius_px)
+    kw["clear_output"] = 1 if clear_output else 0
     shape = (3, height, width) if to_chw else (height, width, 3)
     image = torch.empty(shape, dtype=torch.float32, device=device)
-    v.image = image.data_ptr()
+    kw["image"] = image.data_ptr()
     if blur_periphery:
         temp = torch.empty(shape, dtype=torch.float32, device=device)
         keep.append(temp)
-        v.image_temp = temp.data_ptr()
-    return v, image
+        kw["image_temp"] = temp.data_ptr()
+    return _lib.View(**kw), image
 
 
 def render_views(scene_args: tuple, view_args: list, share_preprocess: bool, clear_output: bool,
