@@ -39,6 +39,9 @@ CASES = [
     Case("bg_tex_raw", 15000, 320, 240, seed=20, background=3, foveation_radius_px=120, blend_width_px=50),
     Case("scale_mod", 20000, 320, 240, seed=21, scale_modifier=1.7),
     Case("empty_scene", 0, 128, 96, seed=22),
+    # the plain GPU drop-in configuration: 1M Gaussians in a 20 m cube, one 1920x1080 view, K=8,
+    # the reference's foveation (375/125 px)
+    Case("plain_1M_1920x1080", 1_000_000, 1920, 1080, seed=2, cube=20.0),
     # more than 65535 small tiles: 32-bit sort keys (end_bit 17) through the whole pipeline
     Case("u32_keys_2048x2080", 20000, 2048, 2080, seed=23, foveation_radius_px=300, blend_width_px=100),
     Case("aaa_fov", 30000, 640, 480, seed=41, aaa=True),
