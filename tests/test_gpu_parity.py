@@ -204,6 +204,75 @@ def test_stereo_shared_preprocess_equals_two_mono(oracle_lib):
         _compare_image("stereo eye", img, o["image"].reshape(case.height, case.width, 3))
 
 
+def _blend_sample_mask(o, w, h, stride):
+    """Pixels the blend writes directly (no upsample mixes into them) in the
+    tiles the oracle blended with `blend_tile_stride`: all pixels of FOVEA
+    sub-tiles, the stride-2 samples of PERIPHERY tiles, the stride-4 samples
+    of QUARTER tiles (BLENDED tiles mix in the upsampled periphery)."""
+    gw = -(-w // 16)
+    part = np.asarray(o["partition"], np.int64).reshape(4, 2)
+    tmp = np.asarray(o["tile_map_part"], np.int64)
+    mask = np.zeros((h, w), bool)
+    for t, step, side in ((0, 1, 8), (1, 2, 8), (3, 4, 4)):
+        for g in range(part[t, 0], part[t, 1]):
+            if g % stride:
+                continue
+            tg = tmp[g]
+            lt, st = tg // 4, tg % 4
+            bx = (lt % gw) * 16 + (st % 2) * 8
+            by = (lt // gw) * 16 + (st // 2) * 8
+            mask[by:by + side * step:step, bx:bx + side * step:step] = True
+    return mask
+
+
+def test_stereo_configuration_matches_oracle(oracle_lib):
+    """The stereo configuration the frame rate is measured on, replayed from
+    its captured graph: 3M Gaussians, two 2064x2208 eyes at 100 deg, K=8,
+    three levels (full resolution within 15 deg of the gaze, half to 35 deg,
+    quarter beyond). Every stage of both eyes against the oracle's mono
+    render of each eye, bit for bit, and the left eye's blend output bit for
+    bit on every 8th active tile (the scalar oracle's full blend of one such
+    eye takes minutes)."""
+    from foveatedhtgscudabackend_amd import FoveatedHTGSRasterizer
+    from foveatedhtgscudabackend_amd import _lib
+    from foveatedhtgscudabackend_amd import scene as S
+    from foveatedhtgscudabackend_amd import tables as T
+    from bench import foveation_radii
+    w, h_px, fov = 2064, 2208, 100.0
+    fr, bw, qr = foveation_radii(3, w, fov)
+    sc = S.make_scene(3_000_000, seed=0, cube=20.0)
+    left, right = S.stereo_cameras(w, h_px, fov, seed=1000)
+    gaze = S.seeded_gaze(left, 2000)
+    sl = S.settings_for(left, gaze, K=8, foveation_radius_px=fr, blend_width_px=bw)
+    sr = S.settings_for(right, gaze, K=8, foveation_radius_px=fr, blend_width_px=bw)
+    args = S.scene_to_torch(sc, "cuda")
+    r = FoveatedHTGSRasterizer(fr, bw, qr)
+    h = _lib.handle_for(0)
+    for _ in range(3):  # exact, then captured, then replayed
+        images = r.render_stereo(*args, sl, sr, False, False, False)
+    torch.cuda.synchronize()
+    stride = 8
+    for view, (cam, img) in enumerate(((left, images[0]), (right, images[1]))):
+        words = T.visibility_mask_np(cam.width, cam.height)
+        tabs = (words, T.visibility_area_table_np(words, cam.width, cam.height),
+                T.fovea_area_table_np(T.fovea_radius_tiles_for(cam.width, cam.height, fr, bw)))
+        ov = oracle_lib.view_from_camera(cam, gaze, tabs, K=8, foveation_radius_px=fr, blend_width_px=bw,
+                                         quarter_radius_px=qr)
+        ov["blend_tile_stride"] = stride if view == 0 else 1 << 30
+        o = oracle_lib.run(sc, ov)
+        case = Case(f"stereo_3M_eye{view}", 3_000_000, w, h_px, seed=0)
+        _compare_stages(case, o, h, view)
+        print(f"{case.name}: stages bit-identical", flush=True)
+        if view == 0:
+            m = _blend_sample_mask(o, w, h_px, stride)
+            got = img.cpu().numpy()[m]
+            want = o["image"].reshape(h_px, w, 3)[m]
+            assert m.sum() > 50_000
+            np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32),
+                                          err_msg=f"{case.name}: blended pixels differ")
+            print(f"{case.name}: {int(m.sum())} blended pixels bit-identical", flush=True)
+
+
 @pytest.mark.parametrize("n,key_bytes,end_bit", [
     (0, 2, 12), (1, 2, 12), (4095, 2, 15), (4096, 4, 17), (4097, 4, 17), (6143, 4, 17), (6144, 2, 16),
     (6145, 4, 17), (12_289, 4, 22), (100_000, 2, 16),
