@@ -17,8 +17,8 @@ the reference's 375/125 px; --levels 2 uses the reference's two levels
 Prints one JSON line. `roofline` is the dominant kernel's algorithmic rate
 from HIP events on the render stream, over frames in which each kernel runs
 alone (the stage profile; in the timed frames the two eyes' branches overlap);
-`cpu_baseline` is the scalar oracle on the box's host (one thread) over a
-bounded sample of the same frame.
+`cpu_baseline` is the scalar oracle on the host's cores (one thread) rendering
+one whole stereo frame of the same workload.
 """
 from __future__ import annotations
 
