@@ -280,7 +280,8 @@ def main() -> int:
     if args.cpu_baseline:
         cb = cpu_baseline(sc, (left, right), gaze, radii, args.K, args.cpu_frames)
         cpu = {"value": 1000.0 / cb["frame_ms"], "unit": "stereo frames/s", "cores": 1, "kind": "port",
-               "sample": (f"scalar oracle, {args.cpu_frames} whole stereo frames of the same workload "
+               "sample": (f"scalar oracle, {args.cpu_frames} whole stereo frame{'s' if args.cpu_frames != 1 else ''} "
+                          f"of the same workload "
                           f"({args.gaussians} Gaussians, 2 eyes x {args.width}x{args.height}, K={args.K}, same "
                           f"foveation, every active tile blended); wall {cb['wall_s']:.1f} s; stage_ms per frame, "
                           f"both eyes"),
