@@ -204,19 +204,21 @@ def test_stereo_shared_preprocess_equals_two_mono(oracle_lib):
         _compare_image("stereo eye", img, o["image"].reshape(case.height, case.width, 3))
 
 
-def test_stereo_configuration_matches_oracle(oracle_lib):
+@pytest.mark.parametrize("levels", [3, 2])
+def test_stereo_configuration_matches_oracle(levels, oracle_lib):
     """The stereo configuration the frame rate is measured on, replayed from
     its captured graph: 3M Gaussians, two 2064x2208 eyes at 100 deg, K=8,
-    three levels (full resolution within 15 deg of the gaze, half to 35 deg,
-    quarter beyond). Every stage and the whole image of both eyes against the
-    oracle's mono render of each eye, bit for bit."""
+    with three levels (full resolution within 15 deg of the gaze, half to
+    35 deg, quarter beyond) or the reference's two (375/125 px). Every stage
+    and the whole image of both eyes against the oracle's mono render of each
+    eye, bit for bit."""
     from foveatedhtgscudabackend_amd import FoveatedHTGSRasterizer
     from foveatedhtgscudabackend_amd import _lib
     from foveatedhtgscudabackend_amd import scene as S
     from foveatedhtgscudabackend_amd import tables as T
     from bench import foveation_radii
     w, h_px, fov = 2064, 2208, 100.0
-    fr, bw, qr = foveation_radii(3, w, fov)
+    fr, bw, qr = foveation_radii(levels, w, fov)
     sc = S.make_scene(3_000_000, seed=0, cube=20.0)
     left, right = S.stereo_cameras(w, h_px, fov, seed=1000)
     gaze = S.seeded_gaze(left, 2000)
@@ -235,7 +237,7 @@ def test_stereo_configuration_matches_oracle(oracle_lib):
         ov = oracle_lib.view_from_camera(cam, gaze, tabs, K=8, foveation_radius_px=fr, blend_width_px=bw,
                                          quarter_radius_px=qr)
         o = oracle_lib.run(sc, ov)
-        case = Case(f"stereo_3M_eye{view}", 3_000_000, w, h_px, seed=0)
+        case = Case(f"stereo_3M_{levels}level_eye{view}", 3_000_000, w, h_px, seed=0)
         _compare_stages(case, o, h, view)
         n_diff = _compare_image(case.name, img.cpu().numpy(), o["image"].reshape(h_px, w, 3))
         assert n_diff == 0, f"{case.name}: {n_diff} values not bit-identical to the oracle"
