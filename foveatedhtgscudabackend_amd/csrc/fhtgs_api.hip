@@ -1218,7 +1218,8 @@ int fhtgs_debug_read(fhtgs_handle h, int32_t view, int32_t id, void* dst, int64_
     const int64_t N = sl.n_prims, I = sl.n_instances, na = sl.n_active;
     const void* src = nullptr;
     int64_t bytes = 0;
-    bool widen = false, strided = false;
+    bool widen = false;
+    int field = -1;  // >= 0: this float4 of each 64-B primitive record
     switch (id) {
         case FHTGS_DBG_TILE_NUM: src = sl.vb.tile_num; bytes = (int64_t)sl.vp.n_large * 4; break;
         case FHTGS_DBG_TILE_MAP: src = sl.vb.tile_map; bytes = na * 4; break;
@@ -1234,10 +1235,10 @@ int fhtgs_debug_read(fhtgs_handle h, int32_t view, int32_t id, void* dst, int64_
         }
         case FHTGS_DBG_N_TOUCHED: src = sl.vb.n_touched; bytes = N * 4; break;
         case FHTGS_DBG_BOUNDS: src = sl.vb.bounds; bytes = N * 8; break;
-        case FHTGS_DBG_VPMT1: src = sl.vb.prim + 0; bytes = N * 16; strided = true; break;
-        case FHTGS_DBG_VPMT2: src = sl.vb.prim + 1; bytes = N * 16; strided = true; break;
-        case FHTGS_DBG_VPMT4: src = sl.vb.prim + 2; bytes = N * 16; strided = true; break;
-        case FHTGS_DBG_RGBA: src = sl.vb.prim + 3; bytes = N * 16; strided = true; break;
+        case FHTGS_DBG_VPMT1: src = sl.vb.prim; bytes = N * 16; field = 0; break;
+        case FHTGS_DBG_VPMT2: src = sl.vb.prim; bytes = N * 16; field = 1; break;
+        case FHTGS_DBG_VPMT4: src = sl.vb.prim; bytes = N * 16; field = 2; break;
+        case FHTGS_DBG_RGBA: src = sl.vb.prim; bytes = N * 16; field = 3; break;
         case FHTGS_DBG_KEYS_UNSORTED:
             if (!h->keep_unsorted) return set_err(h, FHTGS_E_INVALID, "FHTGS_OPT_KEEP_UNSORTED is off");
             src = sl.dbg_keys; bytes = I * 4; break;
@@ -1260,9 +1261,20 @@ int fhtgs_debug_read(fhtgs_handle h, int32_t view, int32_t id, void* dst, int64_
             HCHK(hipMemcpy(tmp.data(), src, (size_t)I * 2, hipMemcpyDeviceToHost));
             uint32_t* o = (uint32_t*)dst;
             for (int64_t k = 0; k < I; ++k) o[k] = tmp[(size_t)k];
-        } else if (strided) {
-            // one float4 field out of each 64-B primitive record
-            HCHK(hipMemcpy2D(dst, 16, src, 64, 16, (size_t)N, hipMemcpyDeviceToHost));
+        } else if (field >= 0) {
+            // one float4 out of each 64-B primitive record: whole records in
+            // contiguous chunks, the field picked out on the host (a pitched
+            // copy of millions of 16-B rows is slow)
+            const int64_t chunk = (int64_t)1 << 20;
+            std::vector<float4> tmp((size_t)std::min(N, chunk) * 4);
+            const float4* rec = (const float4*)src;
+            char* o = (char*)dst;
+            for (int64_t r0 = 0; r0 < N; r0 += chunk) {
+                const int64_t nr = std::min(chunk, N - r0);
+                HCHK(hipMemcpy(tmp.data(), rec + 4 * r0, (size_t)nr * 64, hipMemcpyDeviceToHost));
+                for (int64_t k = 0; k < nr; ++k)
+                    std::memcpy(o + (size_t)(r0 + k) * 16, &tmp[(size_t)(4 * k + field)], 16);
+            }
         } else {
             HCHK(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost));
         }

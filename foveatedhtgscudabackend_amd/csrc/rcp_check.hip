@@ -51,7 +51,9 @@ int main() {
         std::memcpy(&x, &first[k], 4);
         std::printf("  x = %a (0x%08x)\n", x, first[k]);
     }
-    hipFree(d_count);
-    hipFree(d_first);
+    if (hipFree(d_count) != hipSuccess || hipFree(d_first) != hipSuccess) {
+        std::printf("rcp_check: hipFree failed\n");
+        return 2;
+    }
     return count == 0 ? 0 : 1;
 }
