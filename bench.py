@@ -39,8 +39,8 @@ FP32_PEAK_TFLOPS = 157.3
 # HBM bytes per blend dispatch (one eye) of the stereo workload at its default
 # size, per foveation level count, from the two calibrated rocprofv3 PMC runs
 # in profiles/stereo_3M_2064x2208_{3,2}level_pmc_traffic.md (FETCH_SIZE +
-# WRITE_SIZE: 350.2 + 15.3 MB, and 402.6 + 31.1 MB); reported only for those.
-BLEND_TRAFFIC_BYTES = {3: 365.5e6, 2: 433.7e6}
+# WRITE_SIZE: 350.8 + 15.3 MB, and 403.2 + 31.1 MB); reported only for those.
+BLEND_TRAFFIC_BYTES = {3: 366.0e6, 2: 434.2e6}
 
 
 def percentile(xs, q):
