@@ -99,6 +99,9 @@ def load() -> ctypes.CDLL:
     lib.orc_result_copy.restype = ctypes.c_int32
     lib.orc_result_free.argtypes = [ctypes.c_void_p]
     lib.orc_result_free.restype = None
+    lib.orc_upsample.argtypes = [ctypes.POINTER(OrcView), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p,
+                                 ctypes.c_int32]
+    lib.orc_upsample.restype = ctypes.c_int32
     lib.orc_build_visibility_sat.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p]
     lib.orc_build_fovea_sat.argtypes = [ctypes.c_int32, ctypes.c_void_p]
     lib.orc_stable_sort_pairs.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32]
@@ -148,28 +151,8 @@ def _ptr(a: np.ndarray) -> int:
     return a.ctypes.data
 
 
-def run(scene: dict, view: dict, stop_after: int = 0, want=None) -> dict:
-    """Runs the oracle. ``scene``: numpy arrays as from scene.make_scene;
-    ``view``: M (>=12 floats), VPM, VPR_inv, cam_position, gaze_position,
-    visibility_mask, visibility_mask_area_table, fovea_mask_area_table
-    (uint32 arrays), width, height, focal_x, focal_y, center_x, center_y,
-    near_plane, far_plane and optional K, active_sh_bases, scale_modifier,
-    to_chw, blur_periphery, foveation_radius_px, blend_width_px,
-    quarter_radius_px (extension, 0 = off),
-    background_model_type / background_model_data."""
-    lib = load()
-    keep = []
-
-    def arr(a, dt):
-        a = np.ascontiguousarray(a, dtype=dt)
-        keep.append(a)
-        return a
-
-    sc = OrcScene()
-    for k in ("positions", "scales", "rotations", "opacities", "sh_0", "sh_rest"):
-        setattr(sc, k, _ptr(arr(scene[k], np.float32)))
-    sc.n_primitives = int(scene["positions"].shape[0])
-    sc.total_sh_bases = int(np.asarray(scene["sh_rest"]).shape[1])
+def _make_view(view: dict, arr) -> OrcView:
+    """OrcView from a view dict (see run); ``arr`` keeps converted arrays alive."""
     v = OrcView()
     M = np.asarray(view["M"], dtype=np.float32).reshape(-1)[:12]
     v.M[:] = list(M)
@@ -199,6 +182,32 @@ def run(scene: dict, view: dict, stop_after: int = 0, want=None) -> dict:
     v.blend_width_px = int(view.get("blend_width_px", 125))
     v.blend_tile_stride = int(view.get("blend_tile_stride", 1))
     v.quarter_radius_px = int(view.get("quarter_radius_px", 0))
+    return v
+
+
+def run(scene: dict, view: dict, stop_after: int = 0, want=None) -> dict:
+    """Runs the oracle. ``scene``: numpy arrays as from scene.make_scene;
+    ``view``: M (>=12 floats), VPM, VPR_inv, cam_position, gaze_position,
+    visibility_mask, visibility_mask_area_table, fovea_mask_area_table
+    (uint32 arrays), width, height, focal_x, focal_y, center_x, center_y,
+    near_plane, far_plane and optional K, active_sh_bases, scale_modifier,
+    to_chw, blur_periphery, foveation_radius_px, blend_width_px,
+    quarter_radius_px (extension, 0 = off),
+    background_model_type / background_model_data."""
+    lib = load()
+    keep = []
+
+    def arr(a, dt):
+        a = np.ascontiguousarray(a, dtype=dt)
+        keep.append(a)
+        return a
+
+    sc = OrcScene()
+    for k in ("positions", "scales", "rotations", "opacities", "sh_0", "sh_rest"):
+        setattr(sc, k, _ptr(arr(scene[k], np.float32)))
+    sc.n_primitives = int(scene["positions"].shape[0])
+    sc.total_sh_bases = int(np.asarray(scene["sh_rest"]).shape[1])
+    v = _make_view(view, arr)
     err = ctypes.create_string_buffer(512)
     r = lib.orc_run(ctypes.byref(sc), ctypes.byref(v), int(stop_after), err, 512)
     if not r:
@@ -217,6 +226,29 @@ def run(scene: dict, view: dict, stop_after: int = 0, want=None) -> dict:
         lib.orc_result_free(r)
     out["count"] = dict(zip(COUNT_NAMES, (int(x) for x in out["counts"])))
     out["stage_ms"] = dict(zip(STAGES, (float(x) for x in out["timings"])))
+    return out
+
+
+def upsample(view: dict, image: np.ndarray, image_temp: np.ndarray = None) -> np.ndarray:
+    """Runs the oracle's upsampling stage alone on caller images (the view
+    dict as for run; only the mask, gaze, radii, size and flags are read).
+    Returns the filled copy of ``image``; with blur_periphery the blur reads
+    ``image_temp``. Images are float32 (H,W,3), or (3,H,W) with to_chw."""
+    lib = load()
+    keep = []
+
+    def arr(a, dt):
+        a = np.ascontiguousarray(a, dtype=dt)
+        keep.append(a)
+        return a
+
+    v = _make_view(view, arr)
+    out = np.ascontiguousarray(image, dtype=np.float32).copy()
+    tmp = None if image_temp is None else np.ascontiguousarray(image_temp, dtype=np.float32)
+    err = ctypes.create_string_buffer(512)
+    rc = lib.orc_upsample(ctypes.byref(v), _ptr(out), None if tmp is None else _ptr(tmp), err, 512)
+    if rc != 0:
+        raise ValueError(err.value.decode())
     return out
 
 

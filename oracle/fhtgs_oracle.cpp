@@ -786,6 +786,140 @@ void put_rgb(float* img, int W, int H, int x, int y, bool chw, const float* rgb)
     }
 }
 
+// Upsampling stage (interpolation.cuh, launched as in inference.cu:398-466):
+// PERIPHERY, BLENDED and QUARTER tiles of the classification `tile_map_part`
+// / `partition`, in place on image_final (blur off) or from `image` (the
+// temp image the non-FOVEA tiles were blended into) to image_final (blur on).
+void upsample_stage(const orc_view& v, const Fov& fov, float gcx, float gcy, int gw,
+                    const std::vector<uint32_t>& tile_map_part, const uint32_t* partition,
+                    float* image_final, float* image) {
+    const int W = v.width, H = v.height;
+    const int n_large = gw * ((H + kTileL - 1) / kTileL);
+    const bool chw = v.to_chw != 0;
+    std::vector<float> smp;
+    const uint32_t pp0 = partition[2 * PERIPHERY], pp1 = partition[2 * PERIPHERY + 1];
+    const uint32_t pb0 = partition[2 * BLENDED], pb1 = partition[2 * BLENDED + 1];
+    const uint32_t pq0 = partition[2 * QUARTER], pq1 = partition[2 * QUARTER + 1];
+    // QUARTER flag per large tile, for the stride-2 sample grids
+    std::vector<uint8_t> quarter_tile;
+    if (fov.Q > 0) {
+        quarter_tile.assign((size_t)n_large, 0);
+        for (uint32_t g = pq0; g < pq1; ++g) quarter_tile[tile_map_part[g] / kNumSub] = 1;
+    }
+    if (!v.blur_periphery) {
+        // bilinear_interpolation (interpolation.cuh:41-146), in place
+        for (uint32_t g = pp0; g < pp1; ++g) {
+            if (v.blend_tile_stride > 1 && g % (uint32_t)v.blend_tile_stride != 0) continue;
+            const uint32_t lt = tile_map_part[g] / kNumSub;
+            const int lx = (int)(lt % gw), ly = (int)(lt / gw);
+            const int bx = lx * kTileL, by = ly * kTileL;
+            load_samples(image_final, v, gw, lx, ly, bx, by, kTileS + 1, kTileS + 1, smp, kStride, &quarter_tile);
+            for (int ty = 0; ty < kTileL; ++ty)
+                for (int tx = 0; tx < kTileL; ++tx) {
+                    const int px = bx + tx, py = by + ty;
+                    if (!(px < W && py < H)) continue;
+                    const int xm = tx % kStride, ym = ty % kStride;
+                    if (xm == 0 && ym == 0) continue;
+                    float rgb[3];
+                    bilinear(smp, kTileS + 1, tx / kStride, ty / kStride, xm, ym, rgb);
+                    for (int q = 0; q < 3; ++q) rgb[q] = fh_saturate(rgb[q]);
+                    store_rgb(image_final, W, H, px, py, chw, rgb);
+                }
+        }
+        // bilinear_interpolation_blended (interpolation.cuh:149-267), in place
+        for (uint32_t g = pb0; g < pb1; ++g) {
+            if (v.blend_tile_stride > 1 && g % (uint32_t)v.blend_tile_stride != 0) continue;
+            const uint32_t tg = tile_map_part[g];
+            const uint32_t lt = tg / kNumSub, st = tg % kNumSub;
+            const int lx = (int)(lt % gw), ly = (int)(lt / gw);
+            const int bx = lx * kTileL + (int)(st % kStride) * kTileS;
+            const int by = ly * kTileL + (int)(st / kStride) * kTileS;
+            const int ns = kTileS / kStride + 1;
+            load_samples(image_final, v, gw, lx, ly, bx, by, ns, ns, smp, kStride, &quarter_tile);
+            for (int ty = 0; ty < kTileS; ++ty)
+                for (int tx = 0; tx < kTileS; ++tx) {
+                    const int px = bx + tx, py = by + ty;
+                    if (!(px < W && py < H)) continue;
+                    const int xm = tx % kStride, ym = ty % kStride;
+                    if (xm == 0 && ym == 0) continue;
+                    float ip[3], ctr[3], rgb[3];
+                    bilinear(smp, ns, tx / kStride, ty / kStride, xm, ym, ip);
+                    const float bf = ring_factor(fov, gcx, gcy, px, py);
+                    sample_rgb(image_final, px, py, W, H, chw, ctr);
+                    if (bf > 0.0f) {
+                        for (int q = 0; q < 3; ++q) rgb[q] = fh_saturate(fma_(ip[q], bf, ctr[q] * (1.0f - bf)));
+                        store_rgb(image_final, W, H, px, py, chw, rgb);
+                    }
+                }
+        }
+        // QUARTER: bilinear fill from the 5x5 stride-4 samples, in place
+        for (uint32_t g = pq0; g < pq1; ++g) {
+            if (v.blend_tile_stride > 1 && g % (uint32_t)v.blend_tile_stride != 0) continue;
+            const uint32_t lt = tile_map_part[g] / kNumSub;
+            const int lx = (int)(lt % gw), ly = (int)(lt / gw);
+            const int bx = lx * kTileL, by = ly * kTileL;
+            const int ns = kTileL / kQuarterStride + 1;
+            load_samples(image_final, v, gw, lx, ly, bx, by, ns, ns, smp, kQuarterStride);
+            for (int ty = 0; ty < kTileL; ++ty)
+                for (int tx = 0; tx < kTileL; ++tx) {
+                    const int px = bx + tx, py = by + ty;
+                    if (!(px < W && py < H)) continue;
+                    const int xm = tx % kQuarterStride, ym = ty % kQuarterStride;
+                    if (xm == 0 && ym == 0) continue;
+                    float rgb[3];
+                    bilinear(smp, ns, tx / kQuarterStride, ty / kQuarterStride, xm, ym, rgb, kQuarterStride);
+                    for (int q = 0; q < 3; ++q) rgb[q] = fh_saturate(rgb[q]);
+                    store_rgb(image_final, W, H, px, py, chw, rgb);
+                }
+        }
+    } else {
+        // interpolate_and_blur (interpolation.cuh:270-413): temp -> final
+        for (uint32_t g = pp0; g < pp1; ++g) {
+            if (v.blend_tile_stride > 1 && g % (uint32_t)v.blend_tile_stride != 0) continue;
+            const uint32_t lt = tile_map_part[g] / kNumSub;
+            const int lx = (int)(lt % gw), ly = (int)(lt / gw);
+            const int bx = lx * kTileL - kStride, by = ly * kTileL - kStride;
+            load_samples(image, v, gw, lx, ly, bx, by, kTileS + 2, kTileS + 2, smp);
+            for (int ty = 0; ty < kTileL; ++ty)
+                for (int tx = 0; tx < kTileL; ++tx) {
+                    const int px = lx * kTileL + tx, py = ly * kTileL + ty;
+                    if (!(px < W && py < H)) continue;
+                    float acc[3], rgb[3];
+                    const float f = blur_sum(smp, kTileS + 2, tx / 2 + 1, ty / 2 + 1, tx, ty, acc);
+                    for (int q = 0; q < 3; ++q) rgb[q] = fh_saturate(acc[q] * f);
+                    store_rgb(image_final, W, H, px, py, chw, rgb);
+                }
+        }
+        // interpolate_and_blur_blended (interpolation.cuh:416-584)
+        for (uint32_t g = pb0; g < pb1; ++g) {
+            if (v.blend_tile_stride > 1 && g % (uint32_t)v.blend_tile_stride != 0) continue;
+            const uint32_t tg = tile_map_part[g];
+            const uint32_t lt = tg / kNumSub, st = tg % kNumSub;
+            const int lx = (int)(lt % gw), ly = (int)(lt / gw);
+            const int ox = lx * kTileL + (int)(st % kStride) * kTileS;
+            const int oy = ly * kTileL + (int)(st / kStride) * kTileS;
+            const int ns = kTileS / kStride + 2;
+            load_samples(image, v, gw, lx, ly, ox - kStride, oy - kStride, ns, ns, smp);
+            for (int ty = 0; ty < kTileS; ++ty)
+                for (int tx = 0; tx < kTileS; ++tx) {
+                    const int px = ox + tx, py = oy + ty;
+                    if (!(px < W && py < H)) continue;
+                    float acc[3], ctr[3], rgb[3];
+                    const float f = blur_sum(smp, ns, tx / 2 + 1, ty / 2 + 1, tx, ty, acc);
+                    const float bf = ring_factor(fov, gcx, gcy, px, py);
+                    sample_rgb(image, px, py, W, H, chw, ctr);
+                    if (bf > 0.0f) {
+                        for (int q = 0; q < 3; ++q)
+                            rgb[q] = fh_saturate(fma_(ctr[q], 1.0f - bf, (acc[q] * f) * bf));
+                    } else {
+                        for (int q = 0; q < 3; ++q) rgb[q] = ctr[q];
+                    }
+                    store_rgb(image_final, W, H, px, py, chw, rgb);
+                }
+        }
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -1107,132 +1241,39 @@ orc_result* orc_run(const orc_scene* scp, const orc_view* vp, int32_t stop_after
     double t7 = now_ms();
     R->timings[6] = t7 - t6;
 
-    // ---- upsample: interpolation.cuh, launched as in inference.cu:398-466
-    std::vector<float> smp;
-    const uint32_t pp0 = R->partition[2 * PERIPHERY], pp1 = R->partition[2 * PERIPHERY + 1];
-    const uint32_t pb0 = R->partition[2 * BLENDED], pb1 = R->partition[2 * BLENDED + 1];
-    const uint32_t pq0 = R->partition[2 * QUARTER], pq1 = R->partition[2 * QUARTER + 1];
-    // QUARTER flag per large tile, for the stride-2 sample grids
-    std::vector<uint8_t> quarter_tile;
-    if (fov.Q > 0) {
-        quarter_tile.assign((size_t)n_large, 0);
-        for (uint32_t g = pq0; g < pq1; ++g) quarter_tile[R->tile_map_part[g] / kNumSub] = 1;
-    }
-    if (!v.blur_periphery) {
-        // bilinear_interpolation (interpolation.cuh:41-146), in place
-        for (uint32_t g = pp0; g < pp1; ++g) {
-            if (v.blend_tile_stride > 1 && g % (uint32_t)v.blend_tile_stride != 0) continue;
-            const uint32_t lt = R->tile_map_part[g] / kNumSub;
-            const int lx = (int)(lt % gw), ly = (int)(lt / gw);
-            const int bx = lx * kTileL, by = ly * kTileL;
-            load_samples(image_final, v, gw, lx, ly, bx, by, kTileS + 1, kTileS + 1, smp, kStride, &quarter_tile);
-            for (int ty = 0; ty < kTileL; ++ty)
-                for (int tx = 0; tx < kTileL; ++tx) {
-                    const int px = bx + tx, py = by + ty;
-                    if (!(px < W && py < H)) continue;
-                    const int xm = tx % kStride, ym = ty % kStride;
-                    if (xm == 0 && ym == 0) continue;
-                    float rgb[3];
-                    bilinear(smp, kTileS + 1, tx / kStride, ty / kStride, xm, ym, rgb);
-                    for (int q = 0; q < 3; ++q) rgb[q] = fh_saturate(rgb[q]);
-                    store_rgb(image_final, W, H, px, py, chw, rgb);
-                }
-        }
-        // bilinear_interpolation_blended (interpolation.cuh:149-267), in place
-        for (uint32_t g = pb0; g < pb1; ++g) {
-            if (v.blend_tile_stride > 1 && g % (uint32_t)v.blend_tile_stride != 0) continue;
-            const uint32_t tg = R->tile_map_part[g];
-            const uint32_t lt = tg / kNumSub, st = tg % kNumSub;
-            const int lx = (int)(lt % gw), ly = (int)(lt / gw);
-            const int bx = lx * kTileL + (int)(st % kStride) * kTileS;
-            const int by = ly * kTileL + (int)(st / kStride) * kTileS;
-            const int ns = kTileS / kStride + 1;
-            load_samples(image_final, v, gw, lx, ly, bx, by, ns, ns, smp, kStride, &quarter_tile);
-            for (int ty = 0; ty < kTileS; ++ty)
-                for (int tx = 0; tx < kTileS; ++tx) {
-                    const int px = bx + tx, py = by + ty;
-                    if (!(px < W && py < H)) continue;
-                    const int xm = tx % kStride, ym = ty % kStride;
-                    if (xm == 0 && ym == 0) continue;
-                    float ip[3], ctr[3], rgb[3];
-                    bilinear(smp, ns, tx / kStride, ty / kStride, xm, ym, ip);
-                    const float bf = ring_factor(fov, gcx, gcy, px, py);
-                    sample_rgb(image_final, px, py, W, H, chw, ctr);
-                    if (bf > 0.0f) {
-                        for (int q = 0; q < 3; ++q) rgb[q] = fh_saturate(fma_(ip[q], bf, ctr[q] * (1.0f - bf)));
-                        store_rgb(image_final, W, H, px, py, chw, rgb);
-                    }
-                }
-        }
-        // QUARTER: bilinear fill from the 5x5 stride-4 samples, in place
-        for (uint32_t g = pq0; g < pq1; ++g) {
-            if (v.blend_tile_stride > 1 && g % (uint32_t)v.blend_tile_stride != 0) continue;
-            const uint32_t lt = R->tile_map_part[g] / kNumSub;
-            const int lx = (int)(lt % gw), ly = (int)(lt / gw);
-            const int bx = lx * kTileL, by = ly * kTileL;
-            const int ns = kTileL / kQuarterStride + 1;
-            load_samples(image_final, v, gw, lx, ly, bx, by, ns, ns, smp, kQuarterStride);
-            for (int ty = 0; ty < kTileL; ++ty)
-                for (int tx = 0; tx < kTileL; ++tx) {
-                    const int px = bx + tx, py = by + ty;
-                    if (!(px < W && py < H)) continue;
-                    const int xm = tx % kQuarterStride, ym = ty % kQuarterStride;
-                    if (xm == 0 && ym == 0) continue;
-                    float rgb[3];
-                    bilinear(smp, ns, tx / kQuarterStride, ty / kQuarterStride, xm, ym, rgb, kQuarterStride);
-                    for (int q = 0; q < 3; ++q) rgb[q] = fh_saturate(rgb[q]);
-                    store_rgb(image_final, W, H, px, py, chw, rgb);
-                }
-        }
-    } else {
-        // interpolate_and_blur (interpolation.cuh:270-413): temp -> final
-        for (uint32_t g = pp0; g < pp1; ++g) {
-            if (v.blend_tile_stride > 1 && g % (uint32_t)v.blend_tile_stride != 0) continue;
-            const uint32_t lt = R->tile_map_part[g] / kNumSub;
-            const int lx = (int)(lt % gw), ly = (int)(lt / gw);
-            const int bx = lx * kTileL - kStride, by = ly * kTileL - kStride;
-            load_samples(image, v, gw, lx, ly, bx, by, kTileS + 2, kTileS + 2, smp);
-            for (int ty = 0; ty < kTileL; ++ty)
-                for (int tx = 0; tx < kTileL; ++tx) {
-                    const int px = lx * kTileL + tx, py = ly * kTileL + ty;
-                    if (!(px < W && py < H)) continue;
-                    float acc[3], rgb[3];
-                    const float f = blur_sum(smp, kTileS + 2, tx / 2 + 1, ty / 2 + 1, tx, ty, acc);
-                    for (int q = 0; q < 3; ++q) rgb[q] = fh_saturate(acc[q] * f);
-                    store_rgb(image_final, W, H, px, py, chw, rgb);
-                }
-        }
-        // interpolate_and_blur_blended (interpolation.cuh:416-584)
-        for (uint32_t g = pb0; g < pb1; ++g) {
-            if (v.blend_tile_stride > 1 && g % (uint32_t)v.blend_tile_stride != 0) continue;
-            const uint32_t tg = R->tile_map_part[g];
-            const uint32_t lt = tg / kNumSub, st = tg % kNumSub;
-            const int lx = (int)(lt % gw), ly = (int)(lt / gw);
-            const int ox = lx * kTileL + (int)(st % kStride) * kTileS;
-            const int oy = ly * kTileL + (int)(st / kStride) * kTileS;
-            const int ns = kTileS / kStride + 2;
-            load_samples(image, v, gw, lx, ly, ox - kStride, oy - kStride, ns, ns, smp);
-            for (int ty = 0; ty < kTileS; ++ty)
-                for (int tx = 0; tx < kTileS; ++tx) {
-                    const int px = ox + tx, py = oy + ty;
-                    if (!(px < W && py < H)) continue;
-                    float acc[3], ctr[3], rgb[3];
-                    const float f = blur_sum(smp, ns, tx / 2 + 1, ty / 2 + 1, tx, ty, acc);
-                    const float bf = ring_factor(fov, gcx, gcy, px, py);
-                    sample_rgb(image, px, py, W, H, chw, ctr);
-                    if (bf > 0.0f) {
-                        for (int q = 0; q < 3; ++q)
-                            rgb[q] = fh_saturate(fma_(ctr[q], 1.0f - bf, (acc[q] * f) * bf));
-                    } else {
-                        for (int q = 0; q < 3; ++q) rgb[q] = ctr[q];
-                    }
-                    store_rgb(image_final, W, H, px, py, chw, rgb);
-                }
-        }
-    }
+    upsample_stage(v, fov, gcx, gcy, gw, R->tile_map_part, R->partition.data(), image_final, image);
     double t8 = now_ms();
     R->timings[7] = t8 - t7;
     return R;
+}
+
+int32_t orc_upsample(const orc_view* vp, float* image, float* image_temp, char* err, int32_t err_len) {
+    auto fail = [&](const char* msg) -> int32_t {
+        if (err && err_len > 0) { std::snprintf(err, (size_t)err_len, "%s", msg); }
+        return -1;
+    };
+    if (!vp || !image) return fail("null argument");
+    const orc_view& v = *vp;
+    if (v.width <= 0 || v.height <= 0) return fail("width and height must be positive");
+    if (v.blur_periphery && !image_temp) return fail("blur_periphery reads image_temp");
+    if (v.quarter_radius_px > 0 &&
+        (v.foveation_radius_px < 0 || v.quarter_radius_px < v.foveation_radius_px || v.blur_periphery))
+        return fail("the quarter radius needs foveation, must be >= its radius and excludes blur");
+    const int W = v.width, H = v.height;
+    const int gw = (W + kTileL - 1) / kTileL, gh = (H + kTileL - 1) / kTileL;
+    Fov fov = fov_params(v.foveation_radius_px, v.blend_width_px, gw, gh);
+    fov.Q = v.quarter_radius_px > 0 ? (v.quarter_radius_px + kTileL - 1) / kTileL : 0;
+    const float gcx = fh_clamp(v.gaze_position[0], 0.0f, (float)(W - 1));
+    const float gcy = fh_clamp(v.gaze_position[1], 0.0f, (float)(H - 1));
+    int gtx = 0, gty = 0;
+    gaze_tile(v.gaze_position[0], v.gaze_position[1], W, H, gw, gh, gtx, gty);
+    std::vector<uint32_t> tile_num, tile_map, tile_map_part;
+    std::vector<uint8_t> tile_type, tile_type_part;
+    uint32_t partition[8];
+    classify_tiles(v.visibility_mask, gw, gh, fov, gtx, gty, tile_num, tile_map, tile_type, tile_map_part,
+                   tile_type_part, partition);
+    upsample_stage(v, fov, gcx, gcy, gw, tile_map_part, partition, image, v.blur_periphery ? image_temp : image);
+    return 0;
 }
 
 int64_t orc_result_bytes(const orc_result* r, int32_t id) {

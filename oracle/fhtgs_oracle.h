@@ -101,6 +101,13 @@ typedef struct orc_result orc_result;
  * 2 = after the sort. Returns NULL and fills err on invalid arguments. */
 orc_result* orc_run(const orc_scene* scene, const orc_view* view, int32_t stop_after,
                     char* err, int32_t err_len);
+/* The upsampling stage alone, on caller images (a test hook for the
+ * interpolation and blur weights): the view's mask, gaze, radii and flags
+ * classify the tiles as orc_run does, then PERIPHERY, BLENDED and QUARTER
+ * tiles are filled in place in `image` (f32[3HW], layout by to_chw) or, with
+ * blur_periphery, blurred from `image_temp` into `image`. Returns 0, or -1
+ * with err filled. Tables other than the mask are not read. */
+int32_t orc_upsample(const orc_view* view, float* image, float* image_temp, char* err, int32_t err_len);
 int64_t orc_result_bytes(const orc_result* r, int32_t id);
 int32_t orc_result_copy(const orc_result* r, int32_t id, void* dst, int64_t dst_bytes);
 void orc_result_free(orc_result* r);
