@@ -81,6 +81,13 @@ def view_struct(device, M, VPM, VPR_inv, cam_position, gaze_position, visibility
         raise ValueError("gaze_position must have 2 elements")
     kw["gaze_x"], kw["gaze_y"] = gaze_position.detach().to("cpu", torch.float32).reshape(-1).tolist()
     gw, gh = (width + 15) // 16, (height + 15) // 16
+    # the library's own up-front checks (fhtgs_validate_view), before any allocation
+    if gw * gh > _lib.MAX_LARGE_TILES:
+        raise NotImplementedError(f"{width}x{height} has {gw * gh} 16-px tiles; at most "
+                                  f"{_lib.MAX_LARGE_TILES} (4096x4096) are supported")
+    if int(foveation_radius_px) >= 0 and int(blend_width_px) <= 16:
+        raise ValueError("blend_width_px must exceed 16 with foveation on (the ring cross-fade "
+                         "divides by blend_width_px - 16)")
     # extension: both summed-area tables None -> built on the device inside
     # the frame (and a None mask means every tile is visible)
     auto = visibility_mask_area_table is None and fovea_mask_area_table is None

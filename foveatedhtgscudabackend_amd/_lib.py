@@ -48,7 +48,9 @@ EXPORTED_SYMBOLS = (
     "fhtgs_debug_read", "fhtgs_debug_counts", "fhtgs_set_option", "fhtgs_stage_times",
     "fhtgs_render_batch", "fhtgs_autotune", "fhtgs_tuned_digit_bits", "fhtgs_get_option",
     "fhtgs_visualize_gaze", "fhtgs_visualize_tile_boundaries", "fhtgs_build_tables",
+    "fhtgs_validate_view",
 )
+MAX_LARGE_TILES = 65536  # 16-px tiles per view (4096 x 4096 pixels)
 
 
 class Scene(ctypes.Structure):
@@ -141,6 +143,8 @@ def load() -> ctypes.CDLL:
         lib.fhtgs_set_option.restype = ctypes.c_int
         lib.fhtgs_stage_times.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
         lib.fhtgs_stage_times.restype = ctypes.c_int
+        lib.fhtgs_validate_view.argtypes = [ctypes.POINTER(View), ctypes.c_char_p, i32]
+        lib.fhtgs_validate_view.restype = ctypes.c_int
         _lib = lib
         return lib
 
@@ -156,6 +160,15 @@ def fovea_radius_tiles(width: int, height: int, foveation_radius_px: int = DEFAU
     if rc != FHTGS_OK:
         raise ValueError("invalid image size")
     return r.value, b.value
+
+
+def validate_view(view: View) -> tuple[int, str]:
+    """(status, reason) of the library's own checks of a view, on the host
+    only (no device call)."""
+    lib = load()
+    msg = ctypes.create_string_buffer(256)
+    rc = lib.fhtgs_validate_view(ctypes.byref(view), msg, 256)
+    return rc, msg.value.decode()
 
 
 class Handle:

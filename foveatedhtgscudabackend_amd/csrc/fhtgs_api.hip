@@ -281,6 +281,12 @@ int validate_view(fhtgs_context* h, const fhtgs_view& v, int idx) {
         return set_err(h, FHTGS_E_INVALID, p + "texture background needs its size");
     if (v.foveation_radius_px >= 0 && v.blend_width_px > v.foveation_radius_px)
         return set_err(h, FHTGS_E_INVALID, p + "blend width exceeds the foveation radius");
+    // the ring's cross-fade divides by blend_width_px - 16 (interpolation.cuh:244-250)
+    if (v.foveation_radius_px >= 0 && v.blend_width_px <= kTileL)
+        return set_err(h, FHTGS_E_INVALID, p + "blend width must exceed 16 px with foveation on");
+    const int64_t n_large = (int64_t)((v.width + kTileL - 1) / kTileL) * ((v.height + kTileL - 1) / kTileL);
+    if (n_large > kMaxLargeTiles)
+        return set_err(h, FHTGS_E_UNSUPPORTED, p + "more than 65536 16-px tiles (above 4096 x 4096 pixels)");
     if (v.quarter_radius_px > 0) {
         if (v.foveation_radius_px < 0 || v.quarter_radius_px < v.foveation_radius_px)
             return set_err(h, FHTGS_E_INVALID, p + "the quarter radius needs foveation and must be >= its radius");
@@ -652,7 +658,7 @@ int render_impl(fhtgs_context* h, const fhtgs_scene* scenes, int n_scenes, const
         if ((rc = ensure(h, slot.fixed, bytes))) return rc;
         carve_fixed(slot.fixed.ptr, N, slot.vp.n_large, slot.vp.n_small, slot.vb, bytes, slot.vp.gw, slot.vp.gh, tR);
         if (slot.auto_tables) {
-            if (slot.vp.n_large > 65536) return set_err(h, FHTGS_E_INVALID, "image too large for device-built tables");
+            if (slot.vp.n_large > kMaxLargeTiles) return set_err(h, FHTGS_E_UNSUPPORTED, "image too large for device-built tables");
             slot.own_mask = views[v].visibility_mask == nullptr;
             if (slot.own_mask) slot.vp.vis_mask = slot.vb.tab_mask;
             slot.vp.vis_sat = slot.vb.tab_vis_sat;
@@ -993,6 +999,15 @@ int render_entry(fhtgs_context* h, const fhtgs_scene* scenes, int n_scenes, cons
 
 }  // namespace
 
+int fhtgs_validate_view(const fhtgs_view* view, char* msg, int32_t msg_len) {
+    if (msg && msg_len > 0) msg[0] = '\0';
+    if (!view) return FHTGS_E_INVALID;
+    fhtgs_context probe;  // host-side only: collects the message, touches no device
+    const int rc = validate_view(&probe, *view, 0);
+    if (rc && msg && msg_len > 0) std::snprintf(msg, (size_t)msg_len, "%s", probe.err.c_str());
+    return rc;
+}
+
 int fhtgs_render(fhtgs_handle h, const fhtgs_scene* scene, const fhtgs_view* views, int32_t n_views,
                  int32_t share_preprocess, void* stream_ptr) {
     if (!h) return FHTGS_E_INVALID;
@@ -1126,7 +1141,7 @@ int fhtgs_build_tables(fhtgs_handle h, const uint8_t* visible, int32_t width, in
     if (width <= 0 || height <= 0) return set_err(h, FHTGS_E_INVALID, "width and height must be positive");
     if (!mask_out && vis_sat_out) return set_err(h, FHTGS_E_INVALID, "the visibility SAT needs the mask output");
     const int gw = (width + kTileL - 1) / kTileL, gh = (height + kTileL - 1) / kTileL;
-    if (gw * gh > 65536) return set_err(h, FHTGS_E_INVALID, "more than 65536 tiles");
+    if (gw * gh > kMaxLargeTiles) return set_err(h, FHTGS_E_UNSUPPORTED, "more than 65536 tiles");
     int R = 0, B = 0;
     fov_radii(width, height, foveation_radius_px, blend_width_px, R, B);
     DeviceScope scope(h->device);
@@ -1161,6 +1176,8 @@ int fhtgs_visualize_tile_boundaries(fhtgs_handle h, float* image, int32_t width,
         return set_err(h, FHTGS_E_INVALID, "null image/mask or bad size");
     if (foveation_radius_px >= 0 && blend_width_px > foveation_radius_px)
         return set_err(h, FHTGS_E_INVALID, "blend width exceeds the foveation radius");
+    if (foveation_radius_px >= 0 && blend_width_px <= kTileL)
+        return set_err(h, FHTGS_E_INVALID, "blend width must exceed 16 px with foveation on");
     DeviceScope scope(h->device);
     if (!scope.ok) return set_err(h, FHTGS_E_HIP, "cannot select the handle's device");
     hipStream_t s = (hipStream_t)stream_ptr;
@@ -1179,7 +1196,7 @@ int fhtgs_visualize_tile_boundaries(fhtgs_handle h, float* image, int32_t width,
     std::memset(&sc, 0, sizeof(sc));
     ViewSlot& sl = h->vis;
     fill_params(sc, v, sl.vp);
-    if (sl.vp.n_large > 65536) return set_err(h, FHTGS_E_INVALID, "image too large for the tile overlay");
+    if (sl.vp.n_large > kMaxLargeTiles) return set_err(h, FHTGS_E_UNSUPPORTED, "image too large for the tile overlay");
     size_t bytes = 0;
     carve_fixed(nullptr, 0, sl.vp.n_large, sl.vp.n_small, sl.vb, bytes);
     int rc = ensure(h, sl.fixed, bytes);

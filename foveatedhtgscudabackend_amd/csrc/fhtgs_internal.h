@@ -25,6 +25,10 @@ constexpr int kTileBlended = 2;
 // 4 (4x4 samples per 16x16 tile, four tiles per wave) and upsampled 4x.
 constexpr int kTileQuarter = 3;
 constexpr int kQuarterStride = 4;
+// Largest 16-px tile grid a view may have (4096 x 4096 pixels): the tile
+// classification runs in one workgroup and the instance kernel stages the
+// visibility bits of every tile in LDS.
+constexpr int kMaxLargeTiles = 65536;
 
 // Sort geometry: 256 threads x 24 keys per onesweep tile. With three
 // workgroups per CU (137 VGPRs) the 4.8 M keys of one eye of a 2064x2208

@@ -64,6 +64,7 @@ __device__ __forceinline__ void block_scan4(const uint32_t c[4], uint32_t e[4], 
 constexpr int kTilesThreads = 1024;
 constexpr int kTilesMaxPer = 64;  // tiles per thread: up to 65536 large tiles
 constexpr int kTilesMaskWords = kTilesThreads * kTilesMaxPer / 32;
+static_assert(kTilesThreads * kTilesMaxPer >= kMaxLargeTiles, "one tile workgroup covers every view the host accepts");
 
 __global__ __launch_bounds__(kTilesThreads) void tiles_kernel(const ViewParams* vpp, ViewBuffers vb) {
     const ViewParams vp = load_view(vpp);
@@ -331,7 +332,7 @@ __global__ __launch_bounds__(1024) void fovea_sat_kernel(int R, uint32_t* sat) {
 
 hipError_t launch_build_tables(const uint8_t* visible, bool pack, int gw, int gh, int R, uint32_t* words,
                                uint32_t* vis_sat, uint32_t* fovea_sat, hipStream_t s) {
-    if (gw * gh > 65536) return hipErrorInvalidValue;
+    if (gw * gh > kMaxLargeTiles) return hipErrorInvalidValue;
     if (pack) pack_mask_kernel<<<1, 1024, 0, s>>>(visible, gw * gh, words);
     if (vis_sat) vis_sat_kernel<<<1, 1024, 0, s>>>(words, gw, gh, vis_sat);
     if (fovea_sat) fovea_sat_kernel<<<1, 1024, 0, s>>>(R, fovea_sat);

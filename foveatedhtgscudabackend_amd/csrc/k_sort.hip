@@ -276,7 +276,7 @@ hipError_t launch_instances_t(const ViewParams* dvp, const ViewBuffers& vb, cons
 hipError_t launch_instances(const ViewParams* dvp, const ViewBuffers& vb, const InstanceBuffers& ib,
                             const SortPlan& plan, uint32_t n, int n_large, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    if (n_large > 65536) return hipErrorInvalidValue;
+    if (n_large > kMaxLargeTiles) return hipErrorInvalidValue;
     const uint32_t pblocks = (n + kPrimBlock - 1) / kPrimBlock;
     const uint32_t blocks = (pblocks + kInstSubBlocks - 1) / kInstSubBlocks;
     const size_t lds = sizeof(uint32_t) * ((size_t)plan.passes * plan.bins_max + (size_t)(n_large + 31) / 32);

@@ -136,6 +136,16 @@ int fhtgs_destroy(fhtgs_handle h);
 const char* fhtgs_last_error(fhtgs_handle h);
 const char* fhtgs_version(void);
 
+/* Checks a view the way fhtgs_render does before it enqueues anything, on
+ * the host only (no device call, no handle): FHTGS_E_INVALID for bad
+ * arguments (including blend_width_px <= 16 with foveation on: the ring's
+ * cross-fade divides by blend_width_px - 16, interpolation.cuh:244-250),
+ * FHTGS_E_UNSUPPORTED for images of more than 65536 16-px tiles (above
+ * 4096 x 4096 pixels; the reference has no such limit, inference.cu:89-105).
+ * msg (may be NULL) receives the reason. No reference counterpart: the
+ * reference validates nothing. */
+int fhtgs_validate_view(const fhtgs_view* view, char* msg, int32_t msg_len);
+
 /* Renders n_views views of one scene, enqueued on `stream`. With
  * share_preprocess != 0 the Gaussians are read once for both views (stereo,
  * n_views <= 2); the views must then agree on active_sh_bases, near/far,

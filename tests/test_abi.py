@@ -89,3 +89,37 @@ def test_binding_rejects_bad_shapes_before_any_gpu_call():
 def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run(["python", os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def _valid_view(**kw):
+    from foveatedhtgscudabackend_amd import _lib
+    v = dict(M=8, VPM=8, VPR_inv=8, cam_position=8, image=8, visibility_mask=8,
+             visibility_mask_area_table=8, fovea_mask_area_table=8, K=8, active_sh_bases=16,
+             width=2064, height=2208, focal_x=866.0, focal_y=866.0, center_x=1032.0, center_y=1104.0,
+             near_plane=0.2, far_plane=1000.0, scale_modifier=1.0, foveation_radius_px=232,
+             blend_width_px=77, quarter_radius_px=606)
+    v.update(kw)
+    return _lib.View(**v)
+
+
+@pytest.mark.parametrize("kw,code,why", [
+    ({}, 0, ""),
+    ({"foveation_radius_px": -1, "blend_width_px": 0, "quarter_radius_px": 0}, 0, ""),
+    ({"blend_width_px": 16}, -1, "blend width"),
+    ({"blend_width_px": -5}, -1, "blend width"),
+    ({"blend_width_px": 300}, -1, "exceeds the foveation radius"),
+    ({"width": 4096, "height": 4096}, 0, ""),
+    ({"width": 4097, "height": 4096}, -5, "65536"),
+    ({"width": 8192, "height": 300}, 0, ""),
+    ({"width": 8193, "height": 8192}, -5, "65536"),
+    ({"quarter_radius_px": 100}, -1, "quarter radius"),
+    ({"image": None}, -1, "null"),
+])
+def test_view_checks_run_on_the_host_before_any_enqueue(kw, code, why):
+    """fhtgs_validate_view is the check fhtgs_render runs first; it needs no
+    device. Images above 65536 16-px tiles fail here with FHTGS_E_UNSUPPORTED,
+    not later inside a frame."""
+    from foveatedhtgscudabackend_amd import _lib
+    rc, msg = _lib.validate_view(_valid_view(**kw))
+    assert rc == code, msg
+    assert why in msg
