@@ -12,6 +12,8 @@ counterpart).
 from ._binding import render_views
 from ._lib import DEFAULT_BLEND_WIDTH_PX, DEFAULT_FOVEATION_RADIUS_PX
 
+__all__ = ["render"]
+
 
 def render(positions, scales, rotations, opacities, sh_0, sh_rest,
            M, VPM, VPR_inv, cam_position, gaze_position, visibility_mask,

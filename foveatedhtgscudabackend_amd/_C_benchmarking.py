@@ -7,6 +7,8 @@ invisible tiles are undefined.
 from ._binding import render_views
 from ._lib import DEFAULT_BLEND_WIDTH_PX, DEFAULT_FOVEATION_RADIUS_PX
 
+__all__ = ["benchmark"]
+
 
 def benchmark(positions, scales, rotations, opacities, sh_0, sh_rest,
               M, VPM, VPR_inv, cam_position, gaze_position, visibility_mask,

@@ -13,6 +13,8 @@ allocated uninitialised like the reference's (``torch::empty``).
 from ._binding import render_views
 from ._lib import DEFAULT_BLEND_WIDTH_PX, DEFAULT_FOVEATION_RADIUS_PX
 
+__all__ = ["render"]
+
 
 def render(positions, scales, rotations, opacities, sh_0, sh_rest, *args,
            foveation_radius_px=DEFAULT_FOVEATION_RADIUS_PX, blend_width_px=DEFAULT_BLEND_WIDTH_PX,

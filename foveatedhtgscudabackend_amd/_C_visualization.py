@@ -14,6 +14,8 @@ import torch
 from . import _lib
 from ._binding import _dev_ptr, _F32, _U32
 
+__all__ = ["visualize_gaze", "visualize_tile_boundaries"]
+
 
 def _image(image: torch.Tensor, to_chw: bool):
     if not isinstance(image, torch.Tensor) or image.dtype != torch.float32 or image.device.type != "cuda":

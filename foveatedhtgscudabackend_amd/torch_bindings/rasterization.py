@@ -14,6 +14,9 @@ import torch
 from .. import _C, _C_benchmarking, _C_stereo, _C_visualization
 from .._lib import DEFAULT_BLEND_WIDTH_PX, DEFAULT_FOVEATION_RADIUS_PX
 
+__all__ = ["BackgroundModel", "GazeVisualizationType", "RasterizerSettings", "FoveatedHTGSRasterizer",
+           "FoveatedHTGSVisualizer"]
+
 
 class BackgroundModel(Enum):
     NONE = 0
