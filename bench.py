@@ -23,6 +23,7 @@ one whole stereo frame of the same workload.
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import math
 import os
@@ -36,11 +37,6 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
-# HBM bytes per blend dispatch (one eye) of the stereo workload at its default
-# size, per foveation level count, from the two calibrated rocprofv3 PMC runs
-# in profiles/stereo_3M_2064x2208_{3,2}level_pmc_traffic.md (FETCH_SIZE +
-# WRITE_SIZE: 350.8 + 15.3 MB, and 403.2 + 31.1 MB); reported only for those.
-BLEND_TRAFFIC_BYTES = {3: 366.0e6, 2: 434.2e6}
 
 
 def percentile(xs, q):
@@ -115,11 +111,107 @@ def cpu_baseline(scene: dict, cams: tuple, gaze, radii: tuple, K: int, frames: i
     return {"frame_ms": wall * 1e3 / frames, "wall_s": wall, "stage_ms": stage_ms}
 
 
+def blend_source_sha() -> str:
+    """First 16 hex digits of the SHA-256 of the blend kernel's own sources."""
+    h = hashlib.sha256()
+    for f in ("k_blend.hip", "fhtgs_cull.h"):
+        with open(os.path.join(ROOT, "foveatedhtgscudabackend_amd", "csrc", f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def traffic_record(levels: int) -> dict | None:
+    """The blend's HBM bytes per dispatch from the committed PMC profile of
+    the stereo workload (profiles/stereo_3M_2064x2208_<L>level_pmc_traffic.md),
+    with the blend source it was measured on."""
+    path = os.path.join(ROOT, "profiles", f"stereo_3M_2064x2208_{levels}level_pmc_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_record:"):
+            kv = dict(tok.split("=", 1) for tok in line.split()[1:])
+            if kv.get("kernel") == "blend_kernel" and int(kv.get("levels", 0)) == levels:
+                return {"bytes": int(kv["bytes_per_dispatch"]), "blend_source": kv["blend_source"],
+                        "profile": os.path.relpath(path, ROOT)}
+    return None
+
+
+def plain_config(rast, scene_t, settings, steps: int, warmup: int, handle, _lib, gaussians: int) -> dict:
+    """The plain one-view call (exact frames, the reference's per-view path)
+    on one configuration: ms per frame, per-stage ms from HIP events and each
+    stage's algorithmic bytes as a fraction of the HBM peak."""
+    import torch
+    handle.set_option(_lib.OPT_FRAME_MODE, _lib.FRAME_EXACT)
+    for _ in range(warmup):
+        rast.render(*scene_t, settings, False, False, False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        rast.render(*scene_t, settings, False, False, False)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    handle.set_option(_lib.OPT_STAGE_TIMING, 1)
+    stages = []
+    for _ in range(5):
+        rast.render(*scene_t, settings, False, False, False)
+        torch.cuda.synchronize()
+        stages.append(handle.stage_times())
+    handle.set_option(_lib.OPT_STAGE_TIMING, 0)
+    st = {k: float(np.mean([x[k] for x in stages])) for k in stages[0]}
+    c = handle.counts(0)
+    b = stage_bytes(c, gaussians, settings.width * settings.height)
+    return {"ms_per_frame": round(ms, 4), "fps": round(1e3 / ms, 1), "instances": c[4], "visible": c[8],
+            "stage_ms": {k: round(v, 4) for k, v in st.items()},
+            "stage_hbm_frac": {k: round(b[k] / (st[k] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                               for k in b if st.get(k, 0) > 0}}
+
+
+PLAIN_CONFIGS = {
+    # the plain CPU configuration and the plain GPU drop-in configuration, no foveation
+    "cfg1": dict(n=100_000, cube=10.0, width=800, height=800, fov=60.0, K=4),
+    "cfg2": dict(n=1_000_000, cube=20.0, width=1920, height=1080, fov=60.0, K=8),
+}
+
+
+def config_scene(name: str, seed: int):
+    from foveatedhtgscudabackend_amd import scene as S
+    c = PLAIN_CONFIGS[name]
+    sc = S.make_scene(c["n"], seed=seed, cube=c["cube"])
+    cam = S.orbit_camera(c["width"], c["height"], c["fov"], seed=seed + 1000)
+    return sc, cam, S.seeded_gaze(cam, seed + 2000)
+
+
+def cpu_config_ms(name: str, seed: int, repeats: int) -> dict:
+    """The scalar oracle (one thread) on a plain configuration: median
+    end-to-end ms of `repeats` runs and the stage ms of the median run."""
+    import oracle
+    from foveatedhtgscudabackend_amd import tables as T
+    c = PLAIN_CONFIGS[name]
+    sc, cam, gaze = config_scene(name, seed)
+    words = T.visibility_mask_np(cam.width, cam.height)
+    tabs = (words, T.visibility_area_table_np(words, cam.width, cam.height),
+            T.fovea_area_table_np(T.fovea_radius_tiles_for(cam.width, cam.height, -1, 125)))
+    view = oracle.view_from_camera(cam, gaze, tabs, K=c["K"], foveation_radius_px=-1)
+    runs = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        o = oracle.run(sc, view, want=("counts",))
+        runs.append(((time.perf_counter() - t0) * 1e3, o["stage_ms"]))
+    runs.sort(key=lambda r: r[0])
+    ms, stage = runs[len(runs) // 2]
+    return {"ms_per_frame": round(ms, 1), "fps": round(1e3 / ms, 4), "runs": repeats, "cores": 1,
+            "stage_ms": {k: round(v, 1) for k, v in stage.items()},
+            "workload": f"{c['n'] / 1e6:g}M Gaussians ({c['cube']:g} m cube), {c['width']}x{c['height']}, "
+                        f"{c['fov']:g} deg, K={c['K']}, no foveation"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--latency-frames", type=int, default=500,
+                    help="frames of the separate loop the p50/p99 latencies come from")
     ap.add_argument("--gaussians", type=int, default=3_000_000)
     ap.add_argument("--width", type=int, default=2064)
     ap.add_argument("--height", type=int, default=2208)
@@ -137,6 +229,10 @@ def main() -> int:
     ap.add_argument("--cpu-baseline", dest="cpu_baseline", action="store_true", default=True)
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
     ap.add_argument("--cpu-frames", type=int, default=1, help="stereo frames the CPU baseline renders")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the plain 100k/800x800 and 1M/1920x1080 configurations (GPU and CPU)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched sub-object of the stereo line")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving sub-object of the stereo line")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run 3 plain device copies of 256 MiB (calibrates FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--workload", choices=("stereo", "batched", "serving"), default="stereo",
@@ -147,6 +243,8 @@ def main() -> int:
     ap.add_argument("--serve-workers", type=int, default=2, help="serving: batches in flight")
     ap.add_argument("--headsets", type=int, default=8, help="serving: headsets (at most 8, one scene each)")
     ap.add_argument("--serve-graph", type=int, default=1, help="serving: replay captured graphs (1) or not (0)")
+    ap.add_argument("--load", type=float, nargs="+", default=None,
+                    help="serving: offered-load multipliers of 90 Hz per headset (capacity sweep)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -191,20 +289,29 @@ def main() -> int:
         return a, b
 
     def run(fn, steps, warmup, mode):
+        """`steps` frames back to back, bracketed by device synchronisation."""
         handle.set_option(_lib.OPT_FRAME_MODE, mode)
         for _ in range(warmup):
             fn()
         torch.cuda.synchronize()
-        lat = []
         t_start = time.perf_counter()
         for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t_start
+
+    def latencies(fn, frames, mode):
+        """Per-frame latency: call to completion, one frame at a time."""
+        handle.set_option(_lib.OPT_FRAME_MODE, mode)
+        fn()
+        torch.cuda.synchronize()
+        lat = []
+        for _ in range(frames):
             s0 = time.perf_counter()
             fn()
             torch.cuda.synchronize()
             lat.append((time.perf_counter() - s0) * 1e3)
-        torch.cuda.synchronize()
-        total = time.perf_counter() - t_start
-        return total, lat
+        return lat
 
     def stage_profile(fn, frames):
         """Per-stage HIP-event times of the batched frame's kernels: the
@@ -231,17 +338,22 @@ def main() -> int:
 
     # batched: both eyes in one call with one shared preprocess, no
     # mid-frame wait, replayed from a captured hipGraph
-    total, lat = run(stereo, args.steps, args.warmup, _lib.FRAME_GRAPH)
-    counts_eye = [handle.counts(0), handle.counts(1)]
+    total = run(stereo, args.steps, args.warmup, _lib.FRAME_GRAPH)
     fps = args.steps / total
+    lat = latencies(stereo, args.latency_frames, _lib.FRAME_GRAPH)
     stages = stage_profile(stereo, max(5, min(args.steps, 20)))
+    # slots 0 and 1 now hold the left and right eye of the stereo frame: read
+    # its counts and blend evaluations before the plain leg reuses slot 0
+    counts_eye = [handle.counts(0), handle.counts(1)]
+    evals = [blend_evaluations(handle, v, counts_eye[v][2]) for v in range(2)]
     plain_fps = None
     plain_p50 = plain_p99 = None
     if not args.no_plain:
         # plain: one view per call, each waiting mid-frame for its counts
         # (the reference's per-view path)
-        ptotal, plat = run(plain, args.steps, args.warmup, _lib.FRAME_EXACT)
+        ptotal = run(plain, args.steps, args.warmup, _lib.FRAME_EXACT)
         plain_fps = args.steps / ptotal
+        plat = latencies(plain, args.latency_frames, _lib.FRAME_EXACT)
         plain_p50, plain_p99 = percentile(plat, 50), percentile(plat, 99)
         handle.set_option(_lib.OPT_FRAME_MODE, _lib.FRAME_GRAPH)
 
@@ -260,21 +372,44 @@ def main() -> int:
     if dom == "blend":
         # VALU-bound: ~28 FP32 operations per (pixel, Gaussian) evaluation to
         # the cutoff test, for every sample of every tile a Gaussian is listed in
-        evals = sum(blend_evaluations(handle, v, counts_eye[v][2]) for v in range(2))
-        achieved = 28.0 * evals / (stage_ms["blend"] * 1e-3) / 1e12
+        e = sum(evals)
+        achieved = 28.0 * e / (stage_ms["blend"] * 1e-3) / 1e12
         default_workload = (args.gaussians, args.width, args.height, args.K, args.fov) == (3_000_000, 2064, 2208, 8, 100.0)
-        roof = {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+        rec = traffic_record(args.levels) if default_workload else None
+        fresh = rec is not None and rec["blend_source"] == blend_source_sha()
+        roof = {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / FP32_PEAK_TFLOPS,
-                "traffic": BLEND_TRAFFIC_BYTES[args.levels] if default_workload else None, "kernel": "blend_kernel",
-                "evaluations": evals,
-                "note": "VALU-issue bound, FP32 without packed instructions (no MFMA); achieved = 28 FP32 ops x "
-                        "sum over tiles of instances x samples / blend time of both eyes, each blend running "
-                        "alone (stage profile, HIP events); traffic = HBM bytes per dispatch (one eye) from the "
-                        "calibrated PMC runs"}
+                "traffic": rec["bytes"] if fresh else None,
+                "traffic_source": (f"{rec['profile']} (blend source {rec['blend_source']}"
+                                   f"{'' if fresh else ', stale: the blend source has changed since'})") if rec else None,
+                "kernel": "blend_kernel", "evaluations": e, "evaluations_per_eye": evals,
+                "note": "FP32 vector ALU bound (no MFMA, no packed FP32); achieved = 28 FP32 ops x sum over tiles "
+                        "of listed instances x samples, left + right eye, / blend time of both eyes with each blend "
+                        "alone (stage profile, HIP events on the render stream, incl. the two work-order kernels); "
+                        "traffic = calibrated FETCH_SIZE + WRITE_SIZE per dispatch (one eye) of the PMC profile "
+                        "named, null when that profile predates the current blend source"}
     else:
         ach = stage_gbs[dom] if stage_gbs.get(dom) else 0.0
         roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": dom}
+
+    del scene_t
+    torch.cuda.empty_cache()
+    configs = None
+    if not args.no_configs:
+        configs = {}
+        for name, c in PLAIN_CONFIGS.items():
+            csc, ccam, cgaze = config_scene(name, args.seed)
+            ct = S.scene_to_torch(csc, dev)
+            cst = S.settings_for(ccam, cgaze, device=dev, K=c["K"], foveation_radius_px=-1)
+            configs[name] = plain_config(FoveatedHTGSRasterizer(-1, 125, 0), ct, cst, 20, 3, handle, _lib, c["n"])
+            configs[name]["workload"] = (f"plain one-view call: {c['n'] / 1e6:g}M Gaussians ({c['cube']:g} m cube), "
+                                         f"{c['width']}x{c['height']}, {c['fov']:g} deg, K={c['K']}, no foveation")
+            del ct
+        handle.set_option(_lib.OPT_FRAME_MODE, _lib.FRAME_GRAPH)
+        torch.cuda.empty_cache()
+    batched = None if args.no_batched else batched_summary(args)
+    serving = None if args.no_serving else serving_summary(args)
 
     cpu = None
     if args.cpu_baseline:
@@ -285,7 +420,12 @@ def main() -> int:
                           f"({args.gaussians} Gaussians, 2 eyes x {args.width}x{args.height}, K={args.K}, same "
                           f"foveation, every active tile blended); wall {cb['wall_s']:.1f} s; stage_ms per frame, "
                           f"both eyes"),
-               "stage_ms": {k: round(v, 2) for k, v in cb["stage_ms"].items()}}
+               "stage_ms": {k: round(v, 2) for k, v in cb["stage_ms"].items()},
+               "host": host_cpu()}
+        if not args.no_configs:
+            # the plain configurations as the measurement plan times them: median of 3, and once
+            cpu["cfg1"] = cpu_config_ms("cfg1", args.seed, 3)
+            cpu["cfg2"] = cpu_config_ms("cfg2", args.seed, 1)
 
     out = {
         "metric": "stereo foveated frames/sec + p50/p99 frame latency, batched vs plain, 1 GPU",
@@ -310,6 +450,7 @@ def main() -> int:
                    "quarter_radius_px": qr, "parallelism": "single GPU"},
         "p50_ms": percentile(lat, 50),
         "p99_ms": percentile(lat, 99),
+        "latency_frames": args.latency_frames,
         "plain_fps": plain_fps,
         "plain_p50_ms": plain_p50,
         "plain_p99_ms": plain_p99,
@@ -319,12 +460,131 @@ def main() -> int:
         "instances_per_eye": [c[4] for c in counts_eye],
         "visible_per_eye": [c[8] for c in counts_eye],
         "active_tiles_per_eye": [c[3] for c in counts_eye],
+        "blend_evaluations_per_eye": evals,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "plain_configs": configs,
+        "batched": batched,
+        "serving": serving,
         "scene_gen_s": round(gen_s, 2),
     }
     print(json.dumps(out))
     return 0
+
+
+def host_cpu() -> str:
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return f"{model}, {os.cpu_count()} logical CPUs visible, 1 thread used"
+
+
+def batched_summary(args) -> dict:
+    """The batched workload (24 view requests over 4 resident scenes,
+    autotuned, one graph per step) against one view per call, compactly."""
+    import torch
+    from foveatedhtgscudabackend_amd import FoveatedHTGSRasterizer, _lib
+    from foveatedhtgscudabackend_amd.batch import autotune, render_batch
+    scenes = batched_scenes(args.seed, args.cube)
+    views = batched_views(args.seed, args.K, args.levels)
+    handle = _lib.handle_for(torch.cuda.current_device())
+    t0 = time.perf_counter()
+    tuned = autotune(scenes, views, frames=2)
+    tune_s = time.perf_counter() - t0
+    handle.set_option(_lib.OPT_FRAME_MODE, _lib.FRAME_GRAPH)
+    for _ in range(3):
+        render_batch(scenes, views)
+    torch.cuda.synchronize()
+    steps = 10
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        render_batch(scenes, views)
+    torch.cuda.synchronize()
+    vps = steps * len(views) / (time.perf_counter() - t0)
+    handle.set_option(_lib.OPT_FRAME_MODE, _lib.FRAME_EXACT)
+    rast = {}
+
+    def plain():
+        for bv in views:
+            key = (bv.foveation_radius_px, bv.blend_width_px, bv.quarter_radius_px)
+            r = rast.setdefault(key, FoveatedHTGSRasterizer(*key))
+            r.benchmark(*scenes[bv.scene], bv.settings, False, False, False)
+    plain()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        plain()
+    torch.cuda.synchronize()
+    plain_vps = 3 * len(views) / (time.perf_counter() - t0)
+    handle.set_option(_lib.OPT_FRAME_MODE, _lib.FRAME_GRAPH)
+    del scenes
+    torch.cuda.empty_cache()
+    return {"workload": "24 view requests (640x480..2064x2208, 60..100 deg, half stereo pairs, half foveated), "
+                        f"{len(views)} views over 4 resident scenes of 0.5/1.5/3/6 M Gaussians, autotuned, "
+                        "one captured graph per step",
+            "views_per_s": round(vps, 1), "plain_views_per_s": round(plain_vps, 1),
+            "batched_vs_plain": round(vps / plain_vps, 3), "autotune": tuned, "autotune_s": round(tune_s, 2)}
+
+
+P99_BUDGET_MS = 1000.0 / 90.0
+
+
+def serving_summary(args) -> dict:
+    """Capacity of the serving loop: 8 headsets (stereo 2064x2208 at 100
+    deg, the three levels) over the 8 resident scenes (24 M Gaussians), at
+    offered loads of `--load` x 90 Hz per headset, served with continuous
+    batching on two workers (batches of one headset, and of up to two) and
+    on the plain per-request path. Capacity = the highest served stereo
+    frames/s among the loads whose p99 latency is within 11.1 ms (90 Hz)."""
+    import torch
+    from foveatedhtgscudabackend_amd import serving as V
+    width, height, fov = 2064, 2208, 100.0
+    scenes = serving_scenes(args.seed, args.cube)
+    radii = foveation_radii(args.levels, width, fov)
+    loads = args.load or (1.0, 1.5, 2.0)
+    duration = 1.0
+    srv = V.Server(scenes, workers=2, max_batch_headsets=2, foveation=radii, keep_images=False)
+    admitted = [h for h in range(8) if srv.admit(h, h, width, height)]
+    sweep, warmed = [], False
+    for m in loads:
+        headsets = [V.HeadsetSim(h, h, seed=args.seed + 700 + h, rate_hz=90.0 * m) for h in admitted]
+        reqs = V.make_requests(headsets, duration, width, height, fov, foveation=radii)
+        row = {"load": m, "offered_per_s": round(len(reqs) / duration, 1)}
+        if not warmed:
+            srv.reset(2)
+            srv.warm_up(reqs)  # singletons and pairs, on both workers
+            warmed = True
+        for b in (1, 2):
+            for r in reqs:
+                r.dropped, r.images, r.t_start, r.t_done = False, None, 0.0, 0.0
+            srv.reset(b)
+            st = srv.run(reqs)
+            row[f"batch{b}"] = {"fps": round(st["frames_per_s"], 1), "p50_ms": round(st["p50_ms"], 3),
+                                "p99_ms": round(st["p99_ms"], 3), "dropped": st["dropped"]}
+        for r in reqs:
+            r.dropped, r.images, r.t_start, r.t_done = False, None, 0.0, 0.0
+        pl = V.run_plain(scenes, srv.admitted, reqs, foveation=radii, keep_images=False)
+        row["plain"] = {"fps": round(pl["frames_per_s"], 1), "p50_ms": round(pl["p50_ms"], 3),
+                        "p99_ms": round(pl["p99_ms"], 3), "dropped": pl["dropped"]}
+        sweep.append(row)
+    cap = {}
+    for key in ("batch1", "batch2", "plain"):
+        ok = [row[key]["fps"] for row in sweep if row[key]["p99_ms"] <= P99_BUDGET_MS]
+        cap[key] = max(ok) if ok else None
+    best = max((cap[k] for k in ("batch1", "batch2") if cap[k]), default=None)
+    del srv, scenes
+    torch.cuda.empty_cache()
+    return {"workload": f"{len(admitted)} headsets x stereo {width}x{height} at {fov:g} deg, three levels, over 8 "
+                        f"resident scenes of 1..6 M Gaussians (24 M), {duration:g} s of requests per load, 2 workers, "
+                        "latest frame wins",
+            "capacity_fps_p99_11ms": cap,
+            "served_vs_plain_at_capacity": round(best / cap["plain"], 3) if best and cap["plain"] else None,
+            "sweep": sweep}
 
 
 def _common(args, metric, value, unit, ms, config, extra):
@@ -337,19 +597,23 @@ def _common(args, metric, value, unit, ms, config, extra):
     return 0
 
 
-def bench_batched(args) -> int:
-    """24 view requests of uneven size (640x480 .. 2064x2208), half of them
-    stereo pairs, half foveated, over 4 resident scenes (0.5/1.5/3/6 M
-    Gaussians), as one autotuned batch replayed from a captured graph,
-    against the plain path (one view per call, exact frames)."""
-    import torch
-    from foveatedhtgscudabackend_amd import FoveatedHTGSRasterizer, _lib
+BATCH_SCENES = (500_000, 1_500_000, 3_000_000, 6_000_000)
+SERVING_SCENES = (1_000_000, 1_500_000, 2_000_000, 2_500_000, 3_000_000, 3_500_000, 4_500_000, 6_000_000)
+
+
+def batched_scenes(seed: int = 0, cube: float = 20.0) -> list:
+    """The batched workload's 4 resident scenes (0.5/1.5/3/6 M Gaussians) on the GPU."""
     from foveatedhtgscudabackend_amd import scene as S
-    from foveatedhtgscudabackend_amd.batch import BatchView, autotune, render_batch
-    rng = np.random.default_rng(args.seed + 4)
-    sizes_m = (500_000, 1_500_000, 3_000_000, 6_000_000)
-    scenes = [S.scene_to_torch(S.make_scene(n, seed=args.seed + 10 + k, cube=args.cube), "cuda")
-              for k, n in enumerate(sizes_m)]
+    return [S.scene_to_torch(S.make_scene(n, seed=seed + 10 + k, cube=cube), "cuda")
+            for k, n in enumerate(BATCH_SCENES)]
+
+
+def batched_views(seed: int = 0, K: int = 8, levels: int = 3) -> list:
+    """24 view requests of uneven size (640x480 .. 2064x2208, 60..100 deg),
+    half stereo pairs (two BatchViews), half foveated, over the 4 scenes."""
+    from foveatedhtgscudabackend_amd import scene as S
+    from foveatedhtgscudabackend_amd.batch import BatchView
+    rng = np.random.default_rng(seed + 4)
     views = []
     for r in range(24):
         w = int(rng.integers(640, 2065)) // 16 * 16
@@ -357,16 +621,36 @@ def bench_batched(args) -> int:
         sc = int(rng.integers(0, 4))
         fov = float(rng.uniform(60.0, 100.0))
         foveated = rng.random() < 0.5
-        fr, bw, qr = foveation_radii(args.levels, w, fov) if foveated else (-1, 125, 0)
+        fr, bw, qr = foveation_radii(levels, w, fov) if foveated else (-1, 125, 0)
         if rng.random() < 0.5:
-            cams = S.stereo_cameras(w, h, fov, seed=args.seed + 3000 + r)
+            cams = S.stereo_cameras(w, h, fov, seed=seed + 3000 + r)
         else:
-            cams = (S.orbit_camera(w, h, fov, seed=args.seed + 3000 + r),)
-        gaze = S.seeded_gaze(cams[0], args.seed + 5000 + r)
+            cams = (S.orbit_camera(w, h, fov, seed=seed + 3000 + r),)
+        gaze = S.seeded_gaze(cams[0], seed + 5000 + r)
         for cam in cams:
-            st = S.settings_for(cam, gaze, K=args.K, foveation_radius_px=fr, blend_width_px=bw)
+            st = S.settings_for(cam, gaze, K=K, foveation_radius_px=fr, blend_width_px=bw)
             views.append(BatchView(sc, st, foveation_radius_px=fr, blend_width_px=bw, quarter_radius_px=qr,
                                    clear_output=False))
+    return views
+
+
+def serving_scenes(seed: int = 0, cube: float = 20.0) -> list:
+    """The serving workload's 8 resident scenes of 1..6 M Gaussians (24 M) on the GPU."""
+    from foveatedhtgscudabackend_amd import scene as S
+    return [S.scene_to_torch(S.make_scene(n, seed=seed + 20 + k, cube=cube), "cuda")
+            for k, n in enumerate(SERVING_SCENES)]
+
+
+def bench_batched(args) -> int:
+    """24 view requests of uneven size (640x480 .. 2064x2208), half of them
+    stereo pairs, half foveated, over 4 resident scenes (0.5/1.5/3/6 M
+    Gaussians), as one autotuned batch replayed from a captured graph,
+    against the plain path (one view per call, exact frames)."""
+    import torch
+    from foveatedhtgscudabackend_amd import FoveatedHTGSRasterizer, _lib
+    from foveatedhtgscudabackend_amd.batch import autotune, render_batch
+    scenes = batched_scenes(args.seed, args.cube)
+    views = batched_views(args.seed, args.K, args.levels)
     handle = _lib.handle_for(torch.cuda.current_device())
     t0 = time.perf_counter()
     tuned = autotune(scenes, views, frames=2)
@@ -407,7 +691,7 @@ def bench_batched(args) -> int:
     cfg = {"workload": f"batched: 24 view requests (640x480..2064x2208, 60..100 deg, half stereo pairs, half foveated "
                        f"({fov_desc}), K={args.K}) over 4 resident scenes of 0.5/1.5/3/6 M Gaussians, one autotuned "
                        "batch per step, graph replay", "views_per_step": n_views,
-           "scenes_m": [n / 1e6 for n in sizes_m], "levels": args.levels, "parallelism": "single GPU"}
+           "scenes_m": [n / 1e6 for n in BATCH_SCENES], "levels": args.levels, "parallelism": "single GPU"}
     return _common(args, "batched views/sec, 1 GPU (vs one view per call)", vps, "views/s", 1e3 * total / args.steps,
                    cfg, {"p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99), "plain_views_per_s": plain_vps,
                          "batched_vs_plain": (vps / plain_vps) if plain_vps else None, "autotune": tuned,
@@ -421,9 +705,8 @@ def bench_serving(args) -> int:
     import torch
     from foveatedhtgscudabackend_amd import scene as S
     from foveatedhtgscudabackend_amd import serving as V
-    sizes = (1_000_000, 1_500_000, 2_000_000, 2_500_000, 3_000_000, 3_500_000, 4_500_000, 6_000_000)
-    scenes = [S.scene_to_torch(S.make_scene(n, seed=args.seed + 20 + k, cube=args.cube), "cuda")
-              for k, n in enumerate(sizes)]
+    sizes = SERVING_SCENES
+    scenes = serving_scenes(args.seed, args.cube)
     headsets = [V.HeadsetSim(h, h, seed=args.seed + 700 + h) for h in range(max(1, min(8, args.headsets)))]
     radii = foveation_radii(args.levels, args.width, args.fov)
     reqs = V.make_requests(headsets, args.duration, args.width, args.height, args.fov, foveation=radii)

@@ -261,16 +261,32 @@ class Server:
         """Renders each headset's request `frames` times on every worker,
         untimed: the first frame sizes the headset's slots (an exact frame),
         the second captures the worker's graph for it, so that no timed
-        request pays for either."""
+        request pays for either. With batches of two or more headsets every
+        pair of headsets is rendered the same way too (a batch's graph
+        depends on which headsets, and so which slots, it holds); larger
+        batch shapes are captured when they first occur."""
         first = {r.headset: r for r in reversed(requests)}
+        shapes = [(h,) for h in sorted(first)]
+        if self.max_batch >= 2:
+            hs = sorted(first)
+            shapes += [(a, b) for i, a in enumerate(hs) for b in hs[i + 1:]]
         t0 = time.perf_counter()
         for w in range(len(self.handles)):
             for _ in range(frames):
-                for hid in sorted(first):
-                    r = first[hid]
-                    self._render(w, [Request(hid, -1, 0.0, r.settings)], t0)
+                for shape in shapes:
+                    self._render(w, [Request(h, -1, 0.0, first[h].settings) for h in shape], t0)
         torch.cuda.synchronize(self.device)
         self.done = []
+
+    def reset(self, max_batch_headsets: int | None = None) -> None:
+        """Makes the server ready for another run (same handles, slots and
+        captured graphs), optionally with another batch limit."""
+        with self.lock:
+            self.pending = {}
+            self.done = []
+            self.closed = False
+            if max_batch_headsets is not None:
+                self.max_batch = int(max_batch_headsets)
 
     def run(self, requests: list) -> dict:
         """Plays the requests at their arrival times and returns statistics."""

@@ -28,3 +28,31 @@ def test_cpu_baseline_renders_whole_stereo_frames(oracle_lib):
         assert k in cb["stage_ms"] and cb["stage_ms"][k] >= 0.0
     # per-frame stage times of both eyes cannot exceed the frame's wall time
     assert sum(cb["stage_ms"].values()) <= cb["frame_ms"] * 1.001
+
+
+def test_traffic_record_is_read_from_the_profile_and_checked_against_the_source(tmp_path, monkeypatch):
+    import bench
+    for levels in (3, 2):
+        rec = bench.traffic_record(levels)
+        assert rec is not None and rec["bytes"] > 1e8 and len(rec["blend_source"]) == 16
+        assert rec["profile"].startswith("profiles/")
+    # a record measured on another blend source is reported as stale: copy the
+    # repository's profile with a different hash into a scratch root
+    (tmp_path / "profiles").mkdir()
+    src = open(os.path.join(ROOT, "profiles", "stereo_3M_2064x2208_3level_pmc_traffic.md")).read()
+    rec = bench.traffic_record(3)
+    (tmp_path / "profiles" / "stereo_3M_2064x2208_3level_pmc_traffic.md").write_text(
+        src.replace(rec["blend_source"], "0" * 16))
+    monkeypatch.setattr(bench, "ROOT", str(tmp_path))
+    assert bench.traffic_record(3)["blend_source"] == "0" * 16
+    assert bench.traffic_record(2) is None
+    monkeypatch.setattr(bench, "ROOT", ROOT)
+    assert bench.blend_source_sha() != "0" * 16
+
+
+def test_cpu_config_leg_times_the_plain_configuration(oracle_lib):
+    import bench
+    r = bench.cpu_config_ms("cfg1", 0, 1)
+    assert r["ms_per_frame"] > 0 and r["runs"] == 1 and r["cores"] == 1
+    assert "800x800" in r["workload"] and "K=4" in r["workload"]
+    assert r["stage_ms"]["blend"] <= r["ms_per_frame"]

@@ -44,6 +44,10 @@ CASES = [
     # the plain GPU drop-in configuration: 1M Gaussians in a 20 m cube, one 1920x1080 view, K=8,
     # the reference's foveation (375/125 px)
     Case("plain_1M_1920x1080", 1_000_000, 1920, 1080, seed=2, cube=20.0),
+    # the plain CPU configuration: 100k Gaussians in a 10 m cube, 800x800 at 60 deg, K=4, no foveation
+    Case("cfg1_100k_800x800_k4_nofov", 100_000, 800, 800, seed=0, cube=10.0, K=4, foveation_radius_px=-1),
+    # the plain GPU configuration without foveation: 1M Gaussians, 20 m cube, 1920x1080 at 60 deg, K=8
+    Case("cfg2_1M_1920x1080_nofov", 1_000_000, 1920, 1080, seed=0, cube=20.0, K=8, foveation_radius_px=-1),
     # more than 65535 small tiles: 32-bit sort keys (end_bit 17) through the whole pipeline
     Case("u32_keys_2048x2080", 20000, 2048, 2080, seed=23, foveation_radius_px=300, blend_width_px=100),
     Case("aaa_fov", 30000, 640, 480, seed=41, aaa=True),
