@@ -39,6 +39,11 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 
 
+def progress(msg: str) -> None:
+    """One line per finished phase on stderr (the JSON line goes to stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def percentile(xs, q):
     return float(np.percentile(np.asarray(xs, dtype=np.float64), q))
 
@@ -338,10 +343,13 @@ def main() -> int:
 
     # batched: both eyes in one call with one shared preprocess, no
     # mid-frame wait, replayed from a captured hipGraph
+    progress(f"scene of {args.gaussians} Gaussians generated in {gen_s:.1f} s")
     total = run(stereo, args.steps, args.warmup, _lib.FRAME_GRAPH)
     fps = args.steps / total
+    progress(f"stereo: {fps:.1f} frames/s over {args.steps} steps")
     lat = latencies(stereo, args.latency_frames, _lib.FRAME_GRAPH)
     stages = stage_profile(stereo, max(5, min(args.steps, 20)))
+    progress(f"stereo latencies ({args.latency_frames} frames) and stage profile done")
     # slots 0 and 1 now hold the left and right eye of the stereo frame: read
     # its counts and blend evaluations before the plain leg reuses slot 0
     counts_eye = [handle.counts(0), handle.counts(1)]
@@ -356,6 +364,7 @@ def main() -> int:
         plat = latencies(plain, args.latency_frames, _lib.FRAME_EXACT)
         plain_p50, plain_p99 = percentile(plat, 50), percentile(plat, 99)
         handle.set_option(_lib.OPT_FRAME_MODE, _lib.FRAME_GRAPH)
+        progress(f"plain: {plain_fps:.1f} stereo frames/s")
 
     stage_ms = {k: float(np.mean([s[k] for s in stages])) for k in stages[0]}
     pixels = args.width * args.height
@@ -406,10 +415,15 @@ def main() -> int:
             configs[name]["workload"] = (f"plain one-view call: {c['n'] / 1e6:g}M Gaussians ({c['cube']:g} m cube), "
                                          f"{c['width']}x{c['height']}, {c['fov']:g} deg, K={c['K']}, no foveation")
             del ct
+            progress(f"plain {name}: {configs[name]['ms_per_frame']} ms per frame")
         handle.set_option(_lib.OPT_FRAME_MODE, _lib.FRAME_GRAPH)
         torch.cuda.empty_cache()
     batched = None if args.no_batched else batched_summary(args)
+    if batched:
+        progress(f"batched: {batched['views_per_s']} views/s ({batched['batched_vs_plain']}x plain)")
     serving = None if args.no_serving else serving_summary(args)
+    if serving:
+        progress(f"serving capacity at p99 <= 11.1 ms: {serving['capacity_fps_p99_11ms']}")
 
     cpu = None
     if args.cpu_baseline:
@@ -422,10 +436,12 @@ def main() -> int:
                           f"both eyes"),
                "stage_ms": {k: round(v, 2) for k, v in cb["stage_ms"].items()},
                "host": host_cpu()}
+        progress(f"CPU baseline: {cb['frame_ms'] / 1e3:.1f} s per stereo frame")
         if not args.no_configs:
             # the plain configurations as the measurement plan times them: median of 3, and once
             cpu["cfg1"] = cpu_config_ms("cfg1", args.seed, 3)
             cpu["cfg2"] = cpu_config_ms("cfg2", args.seed, 1)
+            progress("CPU baselines of the plain configurations done")
 
     out = {
         "metric": "stereo foveated frames/sec + p50/p99 frame latency, batched vs plain, 1 GPU",
@@ -572,6 +588,7 @@ def serving_summary(args) -> dict:
         row["plain"] = {"fps": round(pl["frames_per_s"], 1), "p50_ms": round(pl["p50_ms"], 3),
                         "p99_ms": round(pl["p99_ms"], 3), "dropped": pl["dropped"]}
         sweep.append(row)
+        progress(f"serving load {m}x: {row}")
     cap = {}
     for key in ("batch1", "batch2", "plain"):
         ok = [row[key]["fps"] for row in sweep if row[key]["p99_ms"] <= P99_BUDGET_MS]
