@@ -268,12 +268,12 @@ def test_single_or_no_active_tile_renders_nothing(restore_mode, n_visible):
     vis = np.zeros((4, 4), bool)
     if n_visible:
         vis[3, 3] = True
-    st = S.settings_for(cam, np.array((0.0, 0.0), np.float32), foveation_radius_px=16, blend_width_px=8,
+    st = S.settings_for(cam, np.array((0.0, 0.0), np.float32), foveation_radius_px=32, blend_width_px=20,
                         visible=vis)
     for mode in (lib.FRAME_EXACT, lib.FRAME_DEVICE, lib.FRAME_GRAPH):
         h.set_option(lib.OPT_FRAME_MODE, mode)
         for _ in range(2):
-            img = _render(args, st, 16, 8)
+            img = _render(args, st, 32, 20)
             assert h.counts(0)[3] == n_visible
             assert not img.any(), f"mode {mode}: pixels rendered"
 
