@@ -10,6 +10,8 @@
 // Extension: QUARTER tiles cover their 16x16 large tile at stride 4 (4x4
 // samples), four tiles per wave: 16-lane groups, each walking its own tile's
 // range in batches of 16 through its own quarter of the LDS batch.
+// Per LDS batch each lane first marks the staged Gaussians that can reach its
+// pixel, then evaluates only those, in order (two passes, same arithmetic).
 // Workgroups take their tiles from a work order with the longest instance
 // lists first (blend_order_count_kernel, blend_order_place_kernel).
 #include "fhtgs_device.h"
@@ -114,6 +116,22 @@ __device__ __forceinline__ uint32_t pack_channel(float x) {
     return (uint32_t)fmaf(fminf(fmaxf(x, 0.0f), 1.0f), 255.0f, 0.5f);
 }
 
+// Bound on rho^2 = num / den beyond which a pixel cannot pass the
+// reference's alpha test (inference.cuh:216-224) for this opacity:
+// alpha = min(o exp(-rho^2 / 2), 1) >= 1/255 needs rho^2 <= 2 ln(255 o). The
+// margin of 0.01 covers the rounding of the logarithm, the exponential, the
+// reciprocal and the products (a few ulp each, below 1e-5 in rho^2), so every
+// pair the reference keeps passes `num <= bound * den`. From an opacity of
+// about 0.995 the bound is the reference's own cutoff constant, and that
+// test is then the reference's cutoff test itself. NaN maps to the cutoff
+// constant and -inf to 0 (the selects, unlike min/max, are explicit about
+// NaN), which keeps every pair whose num or den is NaN.
+__device__ __forceinline__ float pixel_cutoff(float opacity) {
+    float c = 2.0f * fh_logf(255.0f * opacity) + 0.01f;
+    c = c < kMaxCutoffSq ? c : kMaxCutoffSq;
+    return c > 0.0f ? c : 0.0f;
+}
+
 struct BlendArgs {
     const uint32_t* frame_info;
     const uint32_t* order;  // work units, longest first (blend_order_place_kernel)
@@ -140,9 +158,10 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n, uint32_t r
 }
 
 template <int K>
-__global__ __launch_bounds__(64) void blend_kernel(const ViewParams* vpp, BlendArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 5 : 1))) void blend_kernel(const ViewParams* vpp, BlendArgs a) {
     const ViewParams vp = load_view(vpp);
     __shared__ float4 sV1[64], sV2[64], sV4[64], sC[64];
+    __shared__ float sCut[64];
     const uint32_t* fi = a.frame_info;
     const uint32_t g0 = xcd_remap(blockIdx.x, gridDim.x, a.xcd_run);
     const uint32_t nq = fi[7] - fi[6];
@@ -218,11 +237,36 @@ __global__ __launch_bounds__(64) void blend_kernel(const ViewParams* vpp, BlendA
             sV2[pos] = V2;
             sV4[pos] = V4;
             sC[pos] = C;
+            sCut[pos] = pixel_cutoff(C.w);
         }
         __syncthreads();
         if (!inside) continue;
+        // Pass 1: which staged Gaussians can reach this pixel, as a bit mask.
+        // The same num and den as below; a cleared bit is a pair that the
+        // reference's cutoff or alpha test would skip (pixel_cutoff).
+        uint64_t reach = 0ull;
         for (int jj = 0; jj < count; ++jj) {
             const uint32_t e = gbase + (uint32_t)jj;
+            const float4 W1 = sV1[e], W2 = sV2[e], W4 = sV4[e];
+            const float Px0 = fmaf(-W4.x, pxf, W1.x), Px1 = fmaf(-W4.y, pxf, W1.y);
+            const float Px2 = fmaf(-W4.z, pxf, W1.z), Px3 = fmaf(-W4.w, pxf, W1.w);
+            const float Py0 = fmaf(-W4.x, pyf, W2.x), Py1 = fmaf(-W4.y, pyf, W2.y);
+            const float Py2 = fmaf(-W4.z, pyf, W2.z), Py3 = fmaf(-W4.w, pyf, W2.w);
+            const v3 m{fmaf(Px3, Py0, -(Px0 * Py3)), fmaf(Px3, Py1, -(Px1 * Py3)),
+                       fmaf(Px3, Py2, -(Px2 * Py3))};
+            const v3 d = cross3(v3{Px0, Px1, Px2}, v3{Py0, Py1, Py2});
+            const float num = dot3(m, m);
+            const float den = dot3(d, d);
+            reach |= (uint64_t)!(num > sCut[e] * den) << jj;
+        }
+        // Pass 2: the reference's evaluation (inference.cuh:200-238) of those
+        // Gaussians, each lane walking its own bits in ascending order, so
+        // every pixel sees the same Gaussians in the same order as a walk
+        // over the whole batch, and the wave runs as many steps as its
+        // busiest lane needs instead of one per staged Gaussian.
+        while (reach != 0ull) {
+            const uint32_t e = gbase + (uint32_t)__builtin_ctzll(reach);
+            reach &= reach - 1ull;
             const float4 W1 = sV1[e], W2 = sV2[e], W4 = sV4[e];
             const float Px0 = fmaf(-W4.x, pxf, W1.x), Px1 = fmaf(-W4.y, pxf, W1.y);
             const float Px2 = fmaf(-W4.z, pxf, W1.z), Px3 = fmaf(-W4.w, pxf, W1.w);
