@@ -69,3 +69,78 @@ FH_HD bool fh_tile_may_touch(const float a1[4], const float a2[4], const float a
     best = fh_fmin(best, q(fh_clamp(rv * v1, u0, u1), v1));
     return best <= det;
 }
+
+// Pixel-level pre-test of the blend's first pass, a superset of the pixels a
+// Gaussian can reach: the support ellipse of the same dual conic at the
+// level c' = c (1 + kPixelLevelRel) + kPixelLevelAbs, with c the smaller of
+// 2 ln(255 opacity) and the reference's cutoff 11.0825 (every pixel the
+// reference keeps has rho^2 <= c up to rounding), its shape matrix grown by
+// kPixelGrowPx^2 I, in coordinates relative to the tile centre (ox, oy).
+// k[0..5] = ellipse centre (cx, cy), the quadratic form q(du, dv) =
+// k2 du^2 + k3 du dv + k4 dv^2 (k2 = M22, k3 = -2 M12, k4 = M11) and its bound
+// k5 = det M; a pixel at offset (u, v) passes when q(u - cx, v - cy) <= k5.
+// k5 = -1 marks a Gaussian without a proper, finite, not too eccentric
+// ellipse: the first pass then uses the exact test. Conservativeness is
+// audited per pixel (tests/cull_audit.cpp), as for the tile cull.
+constexpr float kPixelLevelRel = 0.02f;
+constexpr float kPixelLevelAbs = 0.02f;
+constexpr float kPixelGrowPx = 0.25f;
+
+FH_HD void fh_pixel_conic(const float a1[4], const float a2[4], const float a4[4], float opacity, float ox,
+                          float oy, float k[6]) {
+    k[0] = k[1] = k[2] = k[3] = k[4] = 0.0f;
+    k[5] = -1.0f;
+    float c = 2.0f * fh_logf(opacity * 255.0f);
+    c = c < 11.0825270903f ? c : 11.0825270903f;  // NaN: the cutoff
+    c = fmaf(c, 1.0f + kPixelLevelRel, kPixelLevelAbs);
+    if (!(c > 0.0f)) return;
+    const float s = fmaf(c, fmaf(a4[2], a4[2], fmaf(a4[1], a4[1], a4[0] * a4[0])), -(a4[3] * a4[3]));
+    if (!(s < 0.0f) || !(a4[3] > 0.0f)) return;
+    const float rs = 1.0f / s;
+    const float fc = rs * c, fw = -rs;
+    float b1[4], b2[4];
+    for (int j = 0; j < 4; ++j) {
+        b1[j] = fmaf(-a4[j], ox, a1[j]);
+        b2[j] = fmaf(-a4[j], oy, a2[j]);
+    }
+    auto dual = [&](const float* u, const float* v) {
+        return fmaf(fw, u[3] * v[3], fc * fmaf(u[2], v[2], fmaf(u[1], v[1], u[0] * v[0])));
+    };
+    const float cx = dual(b1, a4), cy = dual(b2, a4);
+    const float g2 = kPixelGrowPx * kPixelGrowPx;
+    const float m11 = fmaf(cx, cx, -dual(b1, b1)) + g2;
+    const float m22 = fmaf(cy, cy, -dual(b2, b2)) + g2;
+    const float m12 = fmaf(cx, cy, -dual(b1, b2));
+    const float det = fmaf(m11, m22, -(m12 * m12));
+    // finite and small enough that q cannot overflow for |u|, |v| <= 16
+    const bool sane = fabsf(cx) < 1e9f && fabsf(cy) < 1e9f && m11 < 1e17f && m22 < 1e17f &&
+                      fabsf(m12) < 1e17f;
+    if (!sane || !(m11 > 0.0f) || !(m22 > 0.0f) || !(det > 1e-4f * (m11 * m22))) return;
+    k[0] = cx;
+    k[1] = cy;
+    k[2] = m22;
+    k[3] = -2.0f * m12;
+    k[4] = m11;
+    k[5] = det;
+}
+
+FH_HD bool fh_pixel_in_conic(const float k[6], float u, float v) {
+    const float du = u - k[0], dv = v - k[1];
+    return fmaf(k[2], du * du, fmaf(k[4], dv * dv, k[3] * (du * dv))) <= k[5];
+}
+
+// Bound on rho^2 = num / den beyond which a pixel cannot pass the
+// reference's alpha test (inference.cuh:216-224) for this opacity:
+// alpha = min(o exp(-rho^2 / 2), 1) >= 1/255 needs rho^2 <= 2 ln(255 o). The
+// margin of 0.01 covers the rounding of the logarithm, the exponential, the
+// reciprocal and the products (a few ulp each, below 1e-5 in rho^2), so every
+// pair the reference keeps passes `!(num > bound * den)`. From an opacity of
+// about 0.995 the bound is the reference's own cutoff constant, and that
+// test is then the reference's cutoff test itself. NaN maps to the cutoff
+// constant and -inf to 0 (the selects, unlike min/max, are explicit about
+// NaN), which keeps every pair whose num or den is NaN.
+FH_HD float fh_pixel_cutoff(float opacity) {
+    float c = 2.0f * fh_logf(255.0f * opacity) + 0.01f;
+    c = c < 11.0825270903f ? c : 11.0825270903f;
+    return c > 0.0f ? c : 0.0f;
+}

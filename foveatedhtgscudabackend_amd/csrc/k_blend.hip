@@ -116,22 +116,6 @@ __device__ __forceinline__ uint32_t pack_channel(float x) {
     return (uint32_t)fmaf(fminf(fmaxf(x, 0.0f), 1.0f), 255.0f, 0.5f);
 }
 
-// Bound on rho^2 = num / den beyond which a pixel cannot pass the
-// reference's alpha test (inference.cuh:216-224) for this opacity:
-// alpha = min(o exp(-rho^2 / 2), 1) >= 1/255 needs rho^2 <= 2 ln(255 o). The
-// margin of 0.01 covers the rounding of the logarithm, the exponential, the
-// reciprocal and the products (a few ulp each, below 1e-5 in rho^2), so every
-// pair the reference keeps passes `num <= bound * den`. From an opacity of
-// about 0.995 the bound is the reference's own cutoff constant, and that
-// test is then the reference's cutoff test itself. NaN maps to the cutoff
-// constant and -inf to 0 (the selects, unlike min/max, are explicit about
-// NaN), which keeps every pair whose num or den is NaN.
-__device__ __forceinline__ float pixel_cutoff(float opacity) {
-    float c = 2.0f * fh_logf(255.0f * opacity) + 0.01f;
-    c = c < kMaxCutoffSq ? c : kMaxCutoffSq;
-    return c > 0.0f ? c : 0.0f;
-}
-
 struct BlendArgs {
     const uint32_t* frame_info;
     const uint32_t* order;  // work units, longest first (blend_order_place_kernel)
@@ -161,7 +145,9 @@ template <int K>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 5 : 1))) void blend_kernel(const ViewParams* vpp, BlendArgs a) {
     const ViewParams vp = load_view(vpp);
     __shared__ float4 sV1[64], sV2[64], sV4[64], sC[64];
-    __shared__ float sCut[64];
+    // first-pass data per staged Gaussian: pixel conic (centre, form) and its
+    // bound, then the exact test's bound on rho^2 (fh_pixel_conic, fh_pixel_cutoff)
+    __shared__ float4 sQ1[64], sQ2[64];
     const uint32_t* fi = a.frame_info;
     const uint32_t g0 = xcd_remap(blockIdx.x, gridDim.x, a.xcd_run);
     const uint32_t nq = fi[7] - fi[6];
@@ -198,6 +184,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 5 :
     const float pxf = (float)px, pyf = (float)py;
     const float tx0 = (float)bx, tx1 = (float)(bx + (cols - 1u) * step);
     const float ty0 = (float)by, ty1 = (float)(by + (cols - 1u) * step);
+    // the tile centre and this lane's pixel relative to it (first pass)
+    const float ox = 0.5f * (tx0 + tx1), oy = 0.5f * (ty0 + ty1);
+    const float pu = pxf - ox, pv = pyf - oy;
     const uint64_t lt_mask = lanemask_lt() & gmask;
 
     float cd[K];
@@ -215,7 +204,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 5 :
         // group's part of the LDS batch
         const uint32_t idx = b0 + li;
         bool keep = false;
-        float4 V1, V2, V4, C;
+        float4 V1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), V2 = V1, V4 = V1, C = V1;
         if (idx < range.y) {
             const uint32_t p = a.vals[idx];
             const float4* rec = a.prim + 4 * (size_t)p;
@@ -230,43 +219,86 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 5 :
         const uint64_t km = __ballot(keep);
         if (km == 0ull) continue;
         const int count = (int)__popcll(km & gmask);
+        // the first pass runs the same number of steps in every lane: the
+        // largest group count of the wave (QUARTER groups differ)
+        int nmax = (int)__popcll(km);
+        if (quarter) {
+            const int c0 = (int)__popcll(km & 0xffffull), c1 = (int)__popcll((km >> 16) & 0xffffull);
+            const int c2 = (int)__popcll((km >> 32) & 0xffffull), c3 = (int)__popcll(km >> 48);
+            nmax = max(max(c0, c1), max(c2, c3));
+        }
         __syncthreads();
+        const uint32_t pos = gbase + (uint32_t)__popcll(km & lt_mask);
         if (keep) {
-            const uint32_t pos = gbase + (uint32_t)__popcll(km & lt_mask);
             sV1[pos] = V1;
             sV2[pos] = V2;
             sV4[pos] = V4;
             sC[pos] = C;
-            sCut[pos] = pixel_cutoff(C.w);
+        }
+        __syncthreads();
+        // each staging lane adds its Gaussian's first-pass data, from its own
+        // LDS copy (the record's registers are free by now)
+        if (keep) {
+            const float4 W1 = sV1[pos], W2 = sV2[pos], W4 = sV4[pos];
+            const float o = sC[pos].w;
+            const float a1[4] = {W1.x, W1.y, W1.z, W1.w}, a2[4] = {W2.x, W2.y, W2.z, W2.w};
+            const float a4[4] = {W4.x, W4.y, W4.z, W4.w};
+            float kq[6];
+            fh_pixel_conic(a1, a2, a4, o, ox, oy, kq);
+            sQ1[pos] = make_float4(kq[0], kq[1], kq[2], kq[3]);
+            // w = 1: no usable conic, the exact test decides
+            sQ2[pos] = make_float4(kq[4], kq[5], fh_pixel_cutoff(o), kq[5] < 0.0f ? 1.0f : 0.0f);
+        }
+        if ((int)li >= count) {
+            // a slot past the group's Gaussians (any lane there, staging or
+            // not, since its Gaussian went to a lower slot): a conic no pixel
+            // is inside, for the steps of groups with fewer Gaussians than
+            // the wave's largest group
+            sQ1[gbase + li] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            sQ2[gbase + li] = make_float4(0.0f, -1.0f, 0.0f, 0.0f);
         }
         __syncthreads();
         if (!inside) continue;
-        // Pass 1: which staged Gaussians can reach this pixel, as a bit mask.
-        // The same num and den as below; a cleared bit is a pair that the
-        // reference's cutoff or alpha test would skip (pixel_cutoff).
-        uint64_t reach = 0ull;
-        for (int jj = 0; jj < count; ++jj) {
-            const uint32_t e = gbase + (uint32_t)jj;
-            const float4 W1 = sV1[e], W2 = sV2[e], W4 = sV4[e];
-            const float Px0 = fmaf(-W4.x, pxf, W1.x), Px1 = fmaf(-W4.y, pxf, W1.y);
-            const float Px2 = fmaf(-W4.z, pxf, W1.z), Px3 = fmaf(-W4.w, pxf, W1.w);
-            const float Py0 = fmaf(-W4.x, pyf, W2.x), Py1 = fmaf(-W4.y, pyf, W2.y);
-            const float Py2 = fmaf(-W4.z, pyf, W2.z), Py3 = fmaf(-W4.w, pyf, W2.w);
-            const v3 m{fmaf(Px3, Py0, -(Px0 * Py3)), fmaf(Px3, Py1, -(Px1 * Py3)),
-                       fmaf(Px3, Py2, -(Px2 * Py3))};
-            const v3 d = cross3(v3{Px0, Px1, Px2}, v3{Py0, Py1, Py2});
-            const float num = dot3(m, m);
-            const float den = dot3(d, d);
-            reach |= (uint64_t)!(num > sCut[e] * den) << jj;
-        }
+        // Pass 1: which staged Gaussians can reach this pixel. Per Gaussian
+        // either its pixel conic or, without a usable one, the reference's num
+        // and den against fh_pixel_cutoff; both keep every pair the reference's
+        // cutoff and alpha tests keep. The answers are shifted in, first
+        // Gaussian highest: bit 63 - jj of `reach` is staged Gaussian jj.
+        auto may_reach = [&](uint32_t e) -> uint32_t {
+            const float4 q1 = sQ1[e], q2 = sQ2[e];
+            const float k[6] = {q1.x, q1.y, q1.z, q1.w, q2.x, q2.y};
+            bool r = fh_pixel_in_conic(k, pu, pv);
+            if (__ballot(q2.w != 0.0f) != 0ull) {  // rare, wave-uniform
+                const float4 W1 = sV1[e], W2 = sV2[e], W4 = sV4[e];
+                const float Px0 = fmaf(-W4.x, pxf, W1.x), Px1 = fmaf(-W4.y, pxf, W1.y);
+                const float Px2 = fmaf(-W4.z, pxf, W1.z), Px3 = fmaf(-W4.w, pxf, W1.w);
+                const float Py0 = fmaf(-W4.x, pyf, W2.x), Py1 = fmaf(-W4.y, pyf, W2.y);
+                const float Py2 = fmaf(-W4.z, pyf, W2.z), Py3 = fmaf(-W4.w, pyf, W2.w);
+                const v3 m{fmaf(Px3, Py0, -(Px0 * Py3)), fmaf(Px3, Py1, -(Px1 * Py3)),
+                           fmaf(Px3, Py2, -(Px2 * Py3))};
+                const v3 d = cross3(v3{Px0, Px1, Px2}, v3{Py0, Py1, Py2});
+                const bool ex = !(dot3(m, m) > q2.z * dot3(d, d));
+                r = q2.w != 0.0f ? ex : r;
+            }
+            return r ? 1u : 0u;
+        };
+        const int na = nmax < 32 ? nmax : 32;
+        uint32_t ra = 0u, rb = 0u;
+#pragma unroll 4
+        for (int jj = 0; jj < na; ++jj) ra = ra + ra + may_reach(gbase + (uint32_t)jj);
+#pragma unroll 4
+        for (int jj = 32; jj < nmax; ++jj) rb = rb + rb + may_reach(gbase + (uint32_t)jj);
+        uint64_t reach = (uint64_t)(na > 0 ? ra << (32 - na) : 0u) << 32;
+        if (nmax > 32) reach |= rb << (64 - nmax);
         // Pass 2: the reference's evaluation (inference.cuh:200-238) of those
         // Gaussians, each lane walking its own bits in ascending order, so
         // every pixel sees the same Gaussians in the same order as a walk
         // over the whole batch, and the wave runs as many steps as its
         // busiest lane needs instead of one per staged Gaussian.
         while (reach != 0ull) {
-            const uint32_t e = gbase + (uint32_t)__builtin_ctzll(reach);
-            reach &= reach - 1ull;
+            const uint32_t jj = (uint32_t)__builtin_clzll(reach);
+            reach ^= (1ull << 63) >> jj;
+            const uint32_t e = gbase + jj;
             const float4 W1 = sV1[e], W2 = sV2[e], W4 = sV4[e];
             const float Px0 = fmaf(-W4.x, pxf, W1.x), Px1 = fmaf(-W4.y, pxf, W1.y);
             const float Px2 = fmaf(-W4.z, pxf, W1.z), Px3 = fmaf(-W4.w, pxf, W1.w);

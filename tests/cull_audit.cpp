@@ -1,9 +1,13 @@
 // Audit of the blend's per-tile cull (fh_tile_may_touch): over every (tile,
 // instance) pair of a frame, a Gaussian that the cull drops must reach no
 // sample of the tile, evaluated with the blend's own float32 operations
-// (cutoff, exp, alpha >= 1/255). Input is a binary dump of one oracle frame
-// written by tests/test_cull.py; prints pair counts and the number of
-// violations (must be 0).
+// (cutoff, exp, alpha >= 1/255). Then, per pixel, the blend's first-pass
+// pre-test (fh_pixel_conic / fh_pixel_in_conic, or the exact num/den test
+// against fh_pixel_cutoff for Gaussians without a usable conic) must pass at
+// every sample a kept Gaussian reaches. Input is a binary dump of one oracle
+// frame written by tests/test_cull.py; prints pair counts and the number of
+// violations of each (both must be 0), and the smallest relative slack
+// (det - q) / det of the conic test at a reached sample.
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -32,6 +36,27 @@ static bool reaches(const float* V1, const float* V2, const float* V4, float opa
     return fh_fmin(opacity * G, 1.0f) >= 1.0f / 255.0f;
 }
 
+// the blend's first-pass pre-test at pixel (px, py) of a tile centred at (ox, oy)
+static bool pre_test(const float* k, float cut, const float* V1, const float* V2, const float* V4, float pxf,
+                     float pyf, float ox, float oy, double* slack) {
+    if (k[5] >= 0.0f) {
+        const float u = pxf - ox, v = pyf - oy;
+        const float du = u - k[0], dv = v - k[1];
+        const float q = fmaf(k[2], du * du, fmaf(k[4], dv * dv, k[3] * (du * dv)));
+        if (slack) *slack = ((double)k[5] - (double)q) / (double)k[5];
+        return fh_pixel_in_conic(k, u, v);
+    }
+    float Px[4], Py[4];
+    for (int c = 0; c < 4; ++c) {
+        Px[c] = fmaf(-V4[c], pxf, V1[c]);
+        Py[c] = fmaf(-V4[c], pyf, V2[c]);
+    }
+    float m[3], d[3];
+    for (int c = 0; c < 3; ++c) m[c] = fmaf(Px[3], Py[c], -(Px[c] * Py[3]));
+    cross3(Px, Py, d);
+    return !(dot3(m, m) > cut * dot3(d, d));
+}
+
 template <class T>
 static bool rd(FILE* f, std::vector<T>& v, size_t n) {
     v.resize(n);
@@ -54,6 +79,8 @@ int main(int argc, char** argv) {
     }
     std::fclose(f);
     int64_t pairs = 0, kept = 0, reach = 0, bad = 0;
+    int64_t px_reach = 0, px_pre = 0, px_bad = 0, px_evals = 0, no_conic = 0;
+    double min_slack = 1e30;
     for (int type = 0; type < 4; ++type) {
         // periphery tiles sample their large tile at stride 2 (8x8 samples),
         // quarter tiles at stride 4 (4x4 samples)
@@ -78,6 +105,30 @@ int main(int argc, char** argv) {
                 ++pairs;
                 kept += keep;
                 reach += any;
+                if (keep) {
+                    const float ox = 0.5f * (x0 + x1), oy = 0.5f * (y0 + y1);
+                    float k[6];
+                    fh_pixel_conic(a1, a2, a4, o, ox, oy, k);
+                    no_conic += k[5] < 0.0f;
+                    const float cut = fh_pixel_cutoff(o);
+                    for (int q = 0; q < side * side; ++q) {
+                        const int px = bx + (q % side) * step, py = by + (q / side) * step;
+                        if (!(px < W && py < H)) continue;
+                        double sl = 1e30;
+                        const bool r = reaches(a1, a2, a4, o, (float)px, (float)py);
+                        const bool pt = pre_test(k, cut, a1, a2, a4, (float)px, (float)py, ox, oy, &sl);
+                        ++px_evals;
+                        px_reach += r;
+                        px_pre += pt;
+                        if (r && k[5] >= 0.0f && sl < min_slack) min_slack = sl;
+                        if (r && !pt) {
+                            if (px_bad < 5)
+                                std::printf("pixel violation: tile %u prim %u px %d py %d k=(%a %a %a %a %a %a)\n", tg,
+                                            p, px, py, k[0], k[1], k[2], k[3], k[4], k[5]);
+                            ++px_bad;
+                        }
+                    }
+                }
                 if (any && !keep) {
                     if (bad < 5)
                         std::printf("violation: tile %u prim %u a4=(%a %a %a %a) o=%a\n", tg, p, a4[0], a4[1],
@@ -89,5 +140,8 @@ int main(int argc, char** argv) {
     }
     std::printf("pairs %lld kept %lld reaching %lld violations %lld\n", (long long)pairs, (long long)kept,
                 (long long)reach, (long long)bad);
-    return bad != 0;
+    std::printf("pixels %lld reached %lld pretest %lld pixel_violations %lld no_conic %lld min_slack %.6g\n",
+                (long long)px_evals, (long long)px_reach, (long long)px_pre, (long long)px_bad, (long long)no_conic,
+                min_slack);
+    return bad != 0 || px_bad != 0;
 }

@@ -1,8 +1,10 @@
-"""The blend's per-tile cull (fh_tile_may_touch in csrc/fhtgs_cull.h) is an
-optimisation that must not change the image: every Gaussian it drops from a
-tile must reach no sample of that tile. tests/cull_audit.cpp checks this over
-every (tile, instance) pair of oracle frames, with the blend's own float32
-operations, and reports how tight the cull is."""
+"""The blend's per-tile cull (fh_tile_may_touch in csrc/fhtgs_cull.h) and its
+per-pixel first-pass pre-test (fh_pixel_conic / fh_pixel_in_conic) are
+optimisations that must not change the image: every Gaussian the cull drops
+from a tile must reach no sample of that tile, and the pre-test must keep
+every sample a kept Gaussian reaches. tests/cull_audit.cpp checks both over
+every (tile, instance) pair and sample of oracle frames, with the blend's own
+float32 operations, and reports how tight they are."""
 import os
 import re
 import subprocess
@@ -64,3 +66,11 @@ def test_cull_drops_only_unreachable(case, oracle_lib, audit_exe, tmp_path):
     assert pairs == o["count"]["instances"] or pairs <= o["count"]["instances"]
     assert reach <= kept <= pairs
     assert pairs > 0
+    # the blend's per-pixel pre-test keeps every sample a kept Gaussian reaches
+    m = re.search(r"pixels (\d+) reached (\d+) pretest (\d+) pixel_violations (\d+) no_conic (\d+) min_slack (\S+)",
+                  r.stdout)
+    assert m, r.stdout
+    evals, reached, pre, pbad = map(int, m.groups()[:4])
+    assert pbad == 0, r.stdout
+    assert reached <= pre <= evals and reached > 0
+    assert float(m.group(6)) > 0.0
