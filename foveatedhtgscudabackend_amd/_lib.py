@@ -48,8 +48,9 @@ EXPORTED_SYMBOLS = (
     "fhtgs_debug_read", "fhtgs_debug_counts", "fhtgs_set_option", "fhtgs_stage_times",
     "fhtgs_render_batch", "fhtgs_autotune", "fhtgs_tuned_digit_bits", "fhtgs_get_option",
     "fhtgs_visualize_gaze", "fhtgs_visualize_tile_boundaries", "fhtgs_build_tables",
-    "fhtgs_validate_view",
+    "fhtgs_validate_view", "fhtgs_sort_pairs_noalloc",
 )
+SORT_NOALLOC_MAX_PAIRS = 1 << 20  # FHTGS_SORT_NOALLOC_MAX_PAIRS
 MAX_LARGE_TILES = 65536  # 16-px tiles per view (4096 x 4096 pixels)
 
 
@@ -145,6 +146,8 @@ def load() -> ctypes.CDLL:
         lib.fhtgs_stage_times.restype = ctypes.c_int
         lib.fhtgs_validate_view.argtypes = [ctypes.POINTER(View), ctypes.c_char_p, i32]
         lib.fhtgs_validate_view.restype = ctypes.c_int
+        lib.fhtgs_sort_pairs_noalloc.argtypes = [vp, vp, i64, i32, i32, vp, vp]
+        lib.fhtgs_sort_pairs_noalloc.restype = ctypes.c_int
         _lib = lib
         return lib
 
@@ -169,6 +172,18 @@ def validate_view(view: View) -> tuple[int, str]:
     msg = ctypes.create_string_buffer(256)
     rc = lib.fhtgs_validate_view(ctypes.byref(view), msg, 256)
     return rc, msg.value.decode()
+
+
+def sort_pairs_noalloc(keys_ptr: int, vals_ptr: int, n: int, key_bytes: int, end_bit: int,
+                       error_ptr: int | None, stream: int) -> None:
+    """fhtgs_sort_pairs_noalloc: the stable radix sort with the library's fixed
+    per-device scratch (no handle, no allocation; enqueued on `stream` of the
+    current device)."""
+    lib = load()
+    rc = lib.fhtgs_sort_pairs_noalloc(keys_ptr, vals_ptr, n, key_bytes, end_bit, error_ptr, stream)
+    if rc != FHTGS_OK:
+        err = RuntimeError if rc == FHTGS_E_HIP else ValueError
+        raise err(f"fhtgs_sort_pairs_noalloc: {lib.fhtgs_last_error(None).decode()} (status {rc})")
 
 
 class Handle:

@@ -166,8 +166,12 @@ std::mutex& device_memory_lock() {
     return m;
 }
 
+// the message of the last failure of a call without a handle, per thread
+thread_local std::string g_no_handle_err;
+
 int set_err(fhtgs_context* h, int code, const std::string& msg) {
     if (h) h->err = msg;
+    else g_no_handle_err = msg;
     return code;
 }
 
@@ -886,7 +890,10 @@ int fhtgs_destroy(fhtgs_handle h) {
     return FHTGS_OK;
 }
 
-const char* fhtgs_last_error(fhtgs_handle h) { return h ? h->err.c_str() : "null handle"; }
+const char* fhtgs_last_error(fhtgs_handle h) {
+    if (h) return h->err.c_str();
+    return g_no_handle_err.empty() ? "null handle" : g_no_handle_err.c_str();
+}
 
 int fhtgs_fovea_radius_tiles(int32_t width, int32_t height, int32_t fr_px, int32_t bw_px,
                              int32_t* fovea_tiles, int32_t* blend_tiles) {
@@ -1298,6 +1305,98 @@ int fhtgs_debug_read(fhtgs_handle h, int32_t view, int32_t id, void* dst, int64_
     }
     HCHK(hipSetDevice(prev));
     if (written) *written = bytes;
+    return FHTGS_OK;
+}
+
+namespace {
+
+// ---- scratch of fhtgs_sort_pairs_noalloc: one fixed array per device ----
+// Alternate keys and values (4-byte keys at most), digit histograms,
+// tickets plus the error word, and the look-back status of up to kMaxPasses
+// passes of 8-bit digits over kNoAllocMaxPairs pairs, each part 256-B aligned.
+constexpr int64_t kNoAllocMaxPairs = FHTGS_SORT_NOALLOC_MAX_PAIRS;
+constexpr int kNoAllocDigitBits = 8;
+constexpr size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr int64_t kNoAllocTiles = (kNoAllocMaxPairs + kSortTile - 1) / kSortTile;
+constexpr size_t kNoAllocOffVals = a256(4 * (size_t)kNoAllocMaxPairs);
+constexpr size_t kNoAllocOffHist = kNoAllocOffVals + a256(4 * (size_t)kNoAllocMaxPairs);
+constexpr size_t kNoAllocOffTickets = kNoAllocOffHist + a256(4 * (size_t)kMaxPasses * kMaxBins);
+constexpr size_t kNoAllocOffStatus = kNoAllocOffTickets + a256(4 * (size_t)(kMaxPasses + 1));
+constexpr size_t kNoAllocBytes =
+    kNoAllocOffStatus + a256(4 * (size_t)kMaxPasses * kNoAllocTiles * (1u << kNoAllocDigitBits));
+static_assert(kNoAllocBytes < (size_t)16 << 20, "the fixed sort scratch stays modest");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+// Device address of `scratch` per device, from hipGetSymbolAddress on the
+// first call on that device (the array's name is not a device address on
+// the host). Every call on a device shares that device's copy.
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_scratch_lock;
+void* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+int scratch_for_device(int dev, void** out) {
+    if (dev < 0 || dev >= kNoAllocMaxDevices)
+        return set_err(nullptr, FHTGS_E_UNSUPPORTED, "device index beyond the fixed-scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_lock);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_err(nullptr, FHTGS_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return FHTGS_OK;
+}
+
+}  // namespace
+
+int fhtgs_sort_pairs_noalloc(void* keys, uint32_t* values, int64_t n, int32_t key_bytes, int32_t end_bit,
+                             uint32_t* error, void* stream_ptr) {
+    fhtgs_context* h = nullptr;  // no handle: messages go to fhtgs_last_error(NULL)
+    if (n < 0 || n > kNoAllocMaxPairs)
+        return set_err(h, FHTGS_E_INVALID,
+                       "n out of range: the fixed scratch holds at most " + std::to_string(kNoAllocMaxPairs) + " pairs");
+    if (key_bytes != 2 && key_bytes != 4) return set_err(h, FHTGS_E_INVALID, "key_bytes must be 2 or 4");
+    if (end_bit < 1 || end_bit > key_bytes * 8) return set_err(h, FHTGS_E_INVALID, "end_bit out of range");
+    if (n > 0 && (!keys || !values)) return set_err(h, FHTGS_E_INVALID, "null keys or values");
+    hipStream_t s = (hipStream_t)stream_ptr;
+    int dev = 0;
+    HCHK(hipGetDevice(&dev));
+    void* base = nullptr;
+    int rc = scratch_for_device(dev, &base);
+    if (rc) return rc;
+    SortPlan plan = make_plan(end_bit, kNoAllocDigitBits);
+    plan.key_bytes = key_bytes;
+    const int64_t tiles = (n + kSortTile - 1) / kSortTile;
+    InstanceBuffers ib;
+    char* b = (char*)base;
+    ib.keys[0] = keys;
+    ib.vals[0] = values;
+    ib.keys[1] = b;
+    ib.vals[1] = (uint32_t*)(b + kNoAllocOffVals);
+    ib.hist = (uint32_t*)(b + kNoAllocOffHist);
+    ib.tickets = (uint32_t*)(b + kNoAllocOffTickets);
+    ib.status = (uint32_t*)(b + kNoAllocOffStatus);
+    ib.capacity = n;
+    ib.tiles_cap = tiles;
+    // histograms, tickets, error word and status start at zero (a fill
+    // kernel, so the call can be captured into a graph)
+    HCHK(launch_sort_clear(ib, plan, tiles, s));
+    if (n > 0) {
+        HCHK(launch_sort_histogram(ib, plan, 0, (uint32_t)n, s));
+        int src = 0;
+        for (int p = 0; p < plan.passes; ++p) {
+            HCHK(launch_sort_pass(ib, plan, p, src, tiles, nullptr, (uint32_t)n, s));
+            src = 1 - src;
+        }
+        if (src == 1) {
+            HCHK(hipMemcpyAsync(keys, ib.keys[1], (size_t)n * key_bytes, hipMemcpyDeviceToDevice, s));
+            HCHK(hipMemcpyAsync(values, ib.vals[1], (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+        }
+    }
+    if (error) HCHK(hipMemcpyAsync(error, ib.tickets + kMaxPasses, sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
     return FHTGS_OK;
 }
 

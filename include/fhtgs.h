@@ -34,6 +34,8 @@
  *                                rasterization/include/utils/buffer_utils.h:10-143
  *   fhtgs_sort_pairs          <- cub::DeviceRadixSort::SortPairs over bits [0,end_bit),
  *                                rasterization/src/inference.cu:267-274
+ *   fhtgs_sort_pairs_noalloc  <- no reference counterpart: the same sort for callers
+ *                                that cannot allocate (fixed per-device scratch)
  *   fhtgs_build_tables        <- the caller-side culling tables the reference takes as
  *                                arguments (torch_bindings/rasterization.py:28-30;
  *                                layouts read by kernel_utils.cuh:168-210)
@@ -189,6 +191,24 @@ int fhtgs_workspace_bytes(fhtgs_handle h, int64_t* bytes);
  * key_bytes 2 or 4, values uint32. Uses the handle's scratch. */
 int fhtgs_sort_pairs(fhtgs_handle h, void* keys, uint32_t* values, int64_t n,
                      int32_t key_bytes, int32_t end_bit, void* stream);
+
+/* No reference counterpart. For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation): the stable LSD radix sort of fhtgs_sort_pairs (pairs sorted
+ * in place on bits [0, end_bit), key_bytes 2 or 4, uint32 values; 8-bit
+ * digits) with every piece of scratch -- alternate keys and values, digit
+ * histograms, tickets, look-back status -- in one fixed device array of the
+ * library (about 8.7 MB on each device the library runs on, held until the
+ * process ends). It needs no handle, allocates nothing, and returns once the
+ * work is enqueued on `stream` (NULL = the default stream) of the current
+ * device; n above FHTGS_SORT_NOALLOC_MAX_PAIRS is rejected with
+ * FHTGS_E_INVALID. `error` (device uint32, may be NULL) receives 0, or a
+ * nonzero value if the sort detects an inconsistency. All calls on one
+ * device share that device's array: calls that overlap on one device (two
+ * streams, two threads) race. Failures are described by fhtgs_last_error(NULL). */
+#define FHTGS_SORT_NOALLOC_MAX_PAIRS (1 << 20)
+int fhtgs_sort_pairs_noalloc(void* keys, uint32_t* values, int64_t n, int32_t key_bytes,
+                             int32_t end_bit, uint32_t* error, void* stream);
 
 /* Culling tables on the device, in the layouts the render reads (the
  * reference takes them from its caller, torch_bindings/rasterization.py:28-30):
