@@ -48,17 +48,24 @@ def percentile(xs, q):
     return float(np.percentile(np.asarray(xs, dtype=np.float64), q))
 
 
+# per visible Gaussian and view the preprocess writes the 64-B record row,
+# the 48-B support-ellipse record and 8 B of tile bounds; the blend reads
+# both records and the 4-B index per listed instance
+VIS_WRITE_BYTES = 64 + 48 + 8
+INSTANCE_READ_BYTES = 4 + 64 + 48
+
+
 def stage_bytes(counts: list, n_gauss: int, pixels: int) -> dict:
     """Algorithmic HBM bytes per stage for one view (per-unit figures of the
     measurement plan; k = key bytes, P = radix passes, I = instances)."""
     N, I, k, P = n_gauss, counts[4], counts[7], counts[15]
     n_small, n_vis, n_valid = counts[2], max(counts[8], 0), max(counts[5], 0)
     return {
-        "preprocess": N * (236 + 4) + n_vis * 72,
+        "preprocess": N * (236 + 4) + n_vis * VIS_WRITE_BYTES,
         "instances": 8 * N + 8 * n_vis + I * (k + 4),
         "sort": I * k + 2 * P * I * (k + 4),
         "ranges": I * k + 8 * n_small,
-        "blend": n_valid * (4 + 64) + 12 * pixels,
+        "blend": n_valid * INSTANCE_READ_BYTES + 12 * pixels,
     }
 
 
@@ -370,7 +377,7 @@ def main() -> int:
     pixels = args.width * args.height
     bytes_eye = [stage_bytes(c, args.gaussians, pixels) for c in counts_eye]
     # the shared preprocess reads the scene once for both eyes
-    pre_bytes = args.gaussians * 236 + sum(args.gaussians * 4 + max(c[8], 0) * 72 for c in counts_eye)
+    pre_bytes = args.gaussians * 236 + sum(args.gaussians * 4 + max(c[8], 0) * VIS_WRITE_BYTES for c in counts_eye)
     stage_bytes_frame = {k: sum(b[k] for b in bytes_eye) for k in bytes_eye[0]}
     stage_bytes_frame["preprocess"] = pre_bytes
     stage_gbs = {k: (stage_bytes_frame[k] / (stage_ms[k] * 1e-3) / 1e9 if stage_ms.get(k, 0) > 0 else None)

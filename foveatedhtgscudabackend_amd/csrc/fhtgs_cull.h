@@ -1,133 +1,49 @@
-// Per-tile culling of Gaussians in the blend, shared by the gfx950 kernel
-// and host-side audits (plain C++ with -ffp-contract=off sees the same
-// float32 operations).
+// Per-tile and per-pixel culling of Gaussians in the blend, shared by the
+// gfx950 kernels and host-side audits (plain C++ with -ffp-contract=off sees
+// the same float32 operations).
 //
 // A Gaussian's pixel alpha is opacity * exp(-rho^2 / 2) with rho^2 the
 // squared distance of the pixel ray from the Gaussian centre in its unit
 // frame (inference.cuh:202-222), so alpha >= 1/255 only inside the screen
-// conic rho^2 <= c, c = 2 ln(255 opacity). With the rows a1, a2, a4 of VPM*T
-// (VPMT1, VPMT2, VPMT4) and D = diag(c, c, c, -1), the dual of that conic is
+// conic rho^2 <= c, c = 2 ln(255 opacity); the reference also drops every
+// pixel with rho^2 > 11.0825 (its cutoff constant). With the rows a1, a2, a4
+// of VPM*T (VPMT1, VPMT2, VPMT4) and D = diag(c, c, c, -1), the dual of that
+// conic is
 //     C* = sum_k f_k a_i,k a_j,k,  f = D / s,  s = sum_k D_k a4_k^2
 // (quadric silhouette bounds, as in preprocess.cuh's bounding box). When
 // s < 0 and a4.w > 0 the conic is an ellipse with centre
 // (cx, cy) = (C*_14, C*_24) and shape matrix
 //     M = [[cx^2 - C*_11, cx cy - C*_12], [cx cy - C*_12, cy^2 - C*_22]],
 // i.e. the points p with (p - c)^T M^-1 (p - c) <= 1; sqrt(M11), sqrt(M22)
-// are its half extents. The test below keeps a Gaussian iff that ellipse,
-// grown by kCullGrowPx (M + g^2 I), meets the sample rectangle grown by
-// kCullMarginPx: first the box test, then the exact minimum of the convex
-// quadratic over the rectangle (centre inside, else the clamped minimum on
-// each edge). Anything that is not a proper ellipse is kept. Dropping a
-// Gaussian that reaches no sample of the tile leaves the image unchanged.
+// are its half extents.
+//
+// The preprocess stores one such ellipse per visible Gaussian and view (the
+// ellipse record, fh_ellipse_record): the support ellipse at the level
+// c' = min(c, 11.0825) (1 + kPixelLevelRel) + kPixelLevelAbs, its shape
+// matrix grown by kPixelGrowPx^2 I -- a superset of every pixel the
+// reference's cutoff and alpha tests keep, up to rounding. The blend then
+// only translates it per tile:
+//   * fh_ellipse_may_touch: does the ellipse meet the tile's sample
+//     rectangle grown by kCullMarginPx (centre inside, else the exact minimum
+//     of the convex quadratic over the rectangle's edges)? Dropping a
+//     Gaussian that reaches no sample of the tile leaves the image unchanged.
+//   * fh_ellipse_pixel_conic / fh_pixel_in_conic: the per-pixel test of the
+//     blend's first pass, in coordinates relative to the tile centre.
+// A record whose ellipse is not proper and finite is flagged: the tile test
+// then keeps the Gaussian and the pixel test, as for a too eccentric
+// ellipse, falls back to the reference's own num/den test against
+// fh_pixel_cutoff. Conservativeness is audited per (tile, Gaussian) pair and
+// per pixel with these functions (tests/cull_audit.cpp).
 #pragma once
 #include "fhtgs_math.h"
 
 constexpr float kCullMarginPx = 0.0625f;
-constexpr float kCullGrowPx = 0.0625f;
-
-FH_HD bool fh_tile_may_touch(const float a1[4], const float a2[4], const float a4[4], float opacity,
-                             float x0, float x1, float y0, float y1) {
-    const float c = 2.0f * fh_logf(opacity * 255.0f);
-    const float s = fmaf(c, fmaf(a4[2], a4[2], fmaf(a4[1], a4[1], a4[0] * a4[0])), -(a4[3] * a4[3]));
-    if (!(s < 0.0f) || !(a4[3] > 0.0f)) return true;
-    const float rs = 1.0f / s;
-    const float fc = rs * c, fw = -rs;
-    // Screen coordinates relative to the rectangle centre (ox, oy): the rows
-    // become a1 - ox a4 and a2 - oy a4. Near the tile the centre offset and
-    // the shape matrix are then small numbers instead of differences of
-    // squares of pixel coordinates in the thousands.
-    const float ox = 0.5f * (x0 + x1), oy = 0.5f * (y0 + y1);
-    const float hx = 0.5f * (x1 - x0) + kCullMarginPx, hy = 0.5f * (y1 - y0) + kCullMarginPx;
-    float b1[4], b2[4];
-    for (int k = 0; k < 4; ++k) {
-        b1[k] = fmaf(-a4[k], ox, a1[k]);
-        b2[k] = fmaf(-a4[k], oy, a2[k]);
-    }
-    // sum_k f_k u_k v_k with f = (fc, fc, fc, fw)
-    auto dual = [&](const float* u, const float* v) {
-        return fmaf(fw, u[3] * v[3], fc * fmaf(u[2], v[2], fmaf(u[1], v[1], u[0] * v[0])));
-    };
-    const float cx = dual(b1, a4), cy = dual(b2, a4);
-    const float g2 = kCullGrowPx * kCullGrowPx;
-    const float m11 = fmaf(cx, cx, -dual(b1, b1)) + g2;
-    const float m22 = fmaf(cy, cy, -dual(b2, b2)) + g2;
-    const float m12 = fmaf(cx, cy, -dual(b1, b2));
-    const float ex = sqrtf(fh_fmax(m11, 0.0f)), ey = sqrtf(fh_fmax(m22, 0.0f));
-    if ((cx + ex < -hx) || (cx - ex > hx) || (cy + ey < -hy) || (cy - ey > hy)) return false;
-    if (cx >= -hx && cx <= hx && cy >= -hy && cy <= hy) return true;
-    const float det = fmaf(m11, m22, -(m12 * m12));
-    if (!(det > 0.0f) || !(m11 > 0.0f) || !(m22 > 0.0f)) return true;
-    // q(u, v) = m22 u^2 - 2 m12 u v + m11 v^2 <= det  <=>  inside the ellipse
-    auto q = [&](float u, float v) { return fmaf(m22, u * u, fmaf(m11, v * v, -2.0f * (m12 * (u * v)))); };
-    const float u0 = -hx - cx, u1 = hx - cx, v0 = -hy - cy, v1 = hy - cy;
-    const float ru = m12 / m11, rv = m12 / m22;
-    // edges x = const: the minimum over v is at v = (m12 / m11) u
-    float best = q(u0, fh_clamp(ru * u0, v0, v1));
-    best = fh_fmin(best, q(u1, fh_clamp(ru * u1, v0, v1)));
-    // edges y = const: the minimum over u is at u = (m12 / m22) v
-    best = fh_fmin(best, q(fh_clamp(rv * v0, u0, u1), v0));
-    best = fh_fmin(best, q(fh_clamp(rv * v1, u0, u1), v1));
-    return best <= det;
-}
-
-// Pixel-level pre-test of the blend's first pass, a superset of the pixels a
-// Gaussian can reach: the support ellipse of the same dual conic at the
-// level c' = c (1 + kPixelLevelRel) + kPixelLevelAbs, with c the smaller of
-// 2 ln(255 opacity) and the reference's cutoff 11.0825 (every pixel the
-// reference keeps has rho^2 <= c up to rounding), its shape matrix grown by
-// kPixelGrowPx^2 I, in coordinates relative to the tile centre (ox, oy).
-// k[0..5] = ellipse centre (cx, cy), the quadratic form q(du, dv) =
-// k2 du^2 + k3 du dv + k4 dv^2 (k2 = M22, k3 = -2 M12, k4 = M11) and its bound
-// k5 = det M; a pixel at offset (u, v) passes when q(u - cx, v - cy) <= k5.
-// k5 = -1 marks a Gaussian without a proper, finite, not too eccentric
-// ellipse: the first pass then uses the exact test. Conservativeness is
-// audited per pixel (tests/cull_audit.cpp), as for the tile cull.
 constexpr float kPixelLevelRel = 0.02f;
 constexpr float kPixelLevelAbs = 0.02f;
 constexpr float kPixelGrowPx = 0.25f;
-
-FH_HD void fh_pixel_conic(const float a1[4], const float a2[4], const float a4[4], float opacity, float ox,
-                          float oy, float k[6]) {
-    k[0] = k[1] = k[2] = k[3] = k[4] = 0.0f;
-    k[5] = -1.0f;
-    float c = 2.0f * fh_logf(opacity * 255.0f);
-    c = c < 11.0825270903f ? c : 11.0825270903f;  // NaN: the cutoff
-    c = fmaf(c, 1.0f + kPixelLevelRel, kPixelLevelAbs);
-    if (!(c > 0.0f)) return;
-    const float s = fmaf(c, fmaf(a4[2], a4[2], fmaf(a4[1], a4[1], a4[0] * a4[0])), -(a4[3] * a4[3]));
-    if (!(s < 0.0f) || !(a4[3] > 0.0f)) return;
-    const float rs = 1.0f / s;
-    const float fc = rs * c, fw = -rs;
-    float b1[4], b2[4];
-    for (int j = 0; j < 4; ++j) {
-        b1[j] = fmaf(-a4[j], ox, a1[j]);
-        b2[j] = fmaf(-a4[j], oy, a2[j]);
-    }
-    auto dual = [&](const float* u, const float* v) {
-        return fmaf(fw, u[3] * v[3], fc * fmaf(u[2], v[2], fmaf(u[1], v[1], u[0] * v[0])));
-    };
-    const float cx = dual(b1, a4), cy = dual(b2, a4);
-    const float g2 = kPixelGrowPx * kPixelGrowPx;
-    const float m11 = fmaf(cx, cx, -dual(b1, b1)) + g2;
-    const float m22 = fmaf(cy, cy, -dual(b2, b2)) + g2;
-    const float m12 = fmaf(cx, cy, -dual(b1, b2));
-    const float det = fmaf(m11, m22, -(m12 * m12));
-    // finite and small enough that q cannot overflow for |u|, |v| <= 16
-    const bool sane = fabsf(cx) < 1e9f && fabsf(cy) < 1e9f && m11 < 1e17f && m22 < 1e17f &&
-                      fabsf(m12) < 1e17f;
-    if (!sane || !(m11 > 0.0f) || !(m22 > 0.0f) || !(det > 1e-4f * (m11 * m22))) return;
-    k[0] = cx;
-    k[1] = cy;
-    k[2] = m22;
-    k[3] = -2.0f * m12;
-    k[4] = m11;
-    k[5] = det;
-}
-
-FH_HD bool fh_pixel_in_conic(const float k[6], float u, float v) {
-    const float du = u - k[0], dv = v - k[1];
-    return fmaf(k[2], du * du, fmaf(k[4], dv * dv, k[3] * (du * dv))) <= k[5];
-}
+// centres beyond this many pixels from the origin are flagged (their
+// float32 position would lose more than 1/256 px)
+constexpr float kEllipseMaxCoord = 65536.0f;
 
 // Bound on rho^2 = num / den beyond which a pixel cannot pass the
 // reference's alpha test (inference.cuh:216-224) for this opacity:
@@ -143,4 +59,106 @@ FH_HD float fh_pixel_cutoff(float opacity) {
     float c = 2.0f * fh_logf(255.0f * opacity) + 0.01f;
     c = c < 11.0825270903f ? c : 11.0825270903f;
     return c > 0.0f ? c : 0.0f;
+}
+
+// The ellipse record, 5 floats, stored as a float4 and a float:
+//   r[0], r[1]  centre (cx, cy) in pixel coordinates
+//   r[2], r[3]  m11, m22 of the grown shape matrix M, or m11 = -1 when the
+//               ellipse is unusable
+//   r[4]        m12
+// `c2` is 2 ln(255 opacity) as fh_logf computes it (the preprocess has it
+// already for the tile bounds). The shape is computed about an anchor, the
+// first estimate of the centre rounded to whole pixels, where the rows
+// a1 - gx a4, a2 - gy a4 and the centre offset are small numbers; only the
+// centre is then moved back to pixel coordinates (one rounding).
+FH_HD void fh_ellipse_record(const float a1[4], const float a2[4], const float a4[4], float c2, float r[5]) {
+    r[0] = r[1] = r[3] = r[4] = 0.0f;
+    r[2] = -1.0f;
+    float c = c2 < 11.0825270903f ? c2 : 11.0825270903f;  // NaN: the cutoff
+    c = fmaf(c, 1.0f + kPixelLevelRel, kPixelLevelAbs);
+    if (!(c > 0.0f)) return;
+    const float s = fmaf(c, fmaf(a4[2], a4[2], fmaf(a4[1], a4[1], a4[0] * a4[0])), -(a4[3] * a4[3]));
+    if (!(s < 0.0f) || !(a4[3] > 0.0f)) return;
+    const float rs = 1.0f / s;
+    const float fc = rs * c, fw = -rs;
+    // sum_k f_k u_k v_k with f = (fc, fc, fc, fw)
+    auto dual = [&](const float* u, const float* v) {
+        return fmaf(fw, u[3] * v[3], fc * fmaf(u[2], v[2], fmaf(u[1], v[1], u[0] * v[0])));
+    };
+    const float gx0 = dual(a1, a4), gy0 = dual(a2, a4);
+    if (!(fabsf(gx0) < kEllipseMaxCoord) || !(fabsf(gy0) < kEllipseMaxCoord)) return;
+    const float gx = rintf(gx0), gy = rintf(gy0);
+    float b1[4], b2[4];
+    for (int j = 0; j < 4; ++j) {
+        b1[j] = fmaf(-a4[j], gx, a1[j]);
+        b2[j] = fmaf(-a4[j], gy, a2[j]);
+    }
+    const float dx = dual(b1, a4), dy = dual(b2, a4);
+    const float g2 = kPixelGrowPx * kPixelGrowPx;
+    const float m11 = fmaf(dx, dx, -dual(b1, b1)) + g2;
+    const float m22 = fmaf(dy, dy, -dual(b2, b2)) + g2;
+    const float m12 = fmaf(dx, dy, -dual(b1, b2));
+    const float det = fmaf(m11, m22, -(m12 * m12));
+    const float cx = gx + dx, cy = gy + dy;
+    // finite, and small enough that the quadratic forms cannot overflow
+    // near the ellipse
+    const bool sane = fabsf(cx) < kEllipseMaxCoord && fabsf(cy) < kEllipseMaxCoord && m11 < 1e17f &&
+                      m22 < 1e17f && fabsf(m12) < 1e17f;
+    if (!sane || !(m11 > 0.0f) || !(m22 > 0.0f) || !(det > 0.0f)) return;
+    r[0] = cx;
+    r[1] = cy;
+    r[2] = m11;
+    r[3] = m22;
+    r[4] = m12;
+}
+
+// Tile test: keep the Gaussian iff its record's ellipse meets the rectangle
+// [x0, x1] x [y0, y1] of the tile's sample positions grown by kCullMarginPx
+// (or the record is flagged unusable). inv11, inv22 are 1 / m11 and 1 / m22,
+// correctly rounded (the kernels use a 3-instruction reciprocal that equals
+// IEEE division on every float).
+FH_HD bool fh_ellipse_may_touch(const float r[5], float inv11, float inv22, float x0, float x1, float y0,
+                                float y1) {
+    const float m11 = r[2], m22 = r[3], m12 = r[4];
+    if (!(m11 > 0.0f)) return true;
+    // coordinates relative to the rectangle centre
+    const float ox = 0.5f * (x0 + x1), oy = 0.5f * (y0 + y1);
+    const float hx = 0.5f * (x1 - x0) + kCullMarginPx, hy = 0.5f * (y1 - y0) + kCullMarginPx;
+    const float cx = r[0] - ox, cy = r[1] - oy;
+    if (cx >= -hx && cx <= hx && cy >= -hy && cy <= hy) return true;
+    const float det = fmaf(m11, m22, -(m12 * m12));
+    // q(u, v) = m22 u^2 - 2 m12 u v + m11 v^2 <= det  <=>  inside the ellipse
+    auto q = [&](float u, float v) { return fmaf(m22, u * u, fmaf(m11, v * v, -2.0f * (m12 * (u * v)))); };
+    const float u0 = -hx - cx, u1 = hx - cx, v0 = -hy - cy, v1 = hy - cy;
+    const float ru = m12 * inv11, rv = m12 * inv22;
+    // edges x = const: the minimum over v is at v = (m12 / m11) u
+    float best = q(u0, fh_clamp(ru * u0, v0, v1));
+    best = fh_fmin(best, q(u1, fh_clamp(ru * u1, v0, v1)));
+    // edges y = const: the minimum over u is at u = (m12 / m22) v
+    best = fh_fmin(best, q(fh_clamp(rv * v0, u0, u1), v0));
+    best = fh_fmin(best, q(fh_clamp(rv * v1, u0, u1), v1));
+    return best <= det;
+}
+
+// Pixel conic of the first pass relative to the tile centre (ox, oy):
+// k[0..5] = ellipse centre (cx, cy), the quadratic form q(du, dv) =
+// k2 du^2 + k3 du dv + k4 dv^2 (k2 = M22, k3 = -2 M12, k4 = M11) and its bound
+// k5 = det M; a pixel at offset (u, v) passes when q(u - cx, v - cy) <= k5.
+// k5 = -1 marks a Gaussian without a usable ellipse (flagged, or so eccentric
+// that det M <= 1e-4 m11 m22): the first pass then uses the exact test.
+FH_HD void fh_ellipse_pixel_conic(const float r[5], float ox, float oy, float k[6]) {
+    const float m11 = r[2], m22 = r[3], m12 = r[4];
+    const float det = fmaf(m11, m22, -(m12 * m12));
+    const bool ok = m11 > 0.0f && det > 1e-4f * (m11 * m22);
+    k[0] = ok ? r[0] - ox : 0.0f;
+    k[1] = ok ? r[1] - oy : 0.0f;
+    k[2] = ok ? m22 : 0.0f;
+    k[3] = ok ? -2.0f * m12 : 0.0f;
+    k[4] = ok ? m11 : 0.0f;
+    k[5] = ok ? det : -1.0f;
+}
+
+FH_HD bool fh_pixel_in_conic(const float k[6], float u, float v) {
+    const float du = u - k[0], dv = v - k[1];
+    return fmaf(k[2], du * du, fmaf(k[4], dv * dv, k[3] * (du * dv))) <= k[5];
 }

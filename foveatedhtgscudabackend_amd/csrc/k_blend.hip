@@ -123,6 +123,8 @@ struct BlendArgs {
     const uint2* ranges;
     const uint32_t* vals;
     const float4* prim;  // 4 float4 per primitive: VPMT1, VPMT2, VPMT4, rgba
+    const float4* ell_a;  // per primitive: support-ellipse record (fh_ellipse_record): centre, m11, m22
+    const float* ell_b;   // per primitive: its m12
     uint32_t xcd_run;
 };
 
@@ -146,7 +148,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 5 :
     const ViewParams vp = load_view(vpp);
     __shared__ float4 sV1[64], sV2[64], sV4[64], sC[64];
     // first-pass data per staged Gaussian: pixel conic (centre, form) and its
-    // bound, then the exact test's bound on rho^2 (fh_pixel_conic, fh_pixel_cutoff)
+    // bound, then the exact test's bound on rho^2 (fh_ellipse_pixel_conic,
+    // fh_pixel_cutoff)
     __shared__ float4 sQ1[64], sQ2[64];
     const uint32_t* fi = a.frame_info;
     const uint32_t g0 = xcd_remap(blockIdx.x, gridDim.x, a.xcd_run);
@@ -199,22 +202,27 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 5 :
     const uint2 range = have ? a.ranges[tg] : make_uint2(0u, 0u);
     for (uint32_t b0 = range.x; __ballot(b0 < range.y) != 0ull; b0 += gsz) {
         // each lane loads one instance of its group's next gsz and tests its
-        // alpha support ellipse against the tile's samples
-        // (fh_tile_may_touch); survivors are compacted in order into the
-        // group's part of the LDS batch
+        // support ellipse (the preprocess's record) against the tile's
+        // samples (fh_ellipse_may_touch); survivors are compacted in order
+        // into the group's part of the LDS batch
         const uint32_t idx = b0 + li;
         bool keep = false;
         float4 V1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), V2 = V1, V4 = V1, C = V1;
+        float er[5] = {0.0f, 0.0f, -1.0f, 0.0f, 0.0f};
         if (idx < range.y) {
             const uint32_t p = a.vals[idx];
+            const float4 E = a.ell_a[p];
+            er[0] = E.x;
+            er[1] = E.y;
+            er[2] = E.z;
+            er[3] = E.w;
+            er[4] = a.ell_b[p];
             const float4* rec = a.prim + 4 * (size_t)p;
             V1 = rec[0];
             V2 = rec[1];
             V4 = rec[2];
             C = rec[3];
-            const float a1[4] = {V1.x, V1.y, V1.z, V1.w}, a2[4] = {V2.x, V2.y, V2.z, V2.w};
-            const float a4[4] = {V4.x, V4.y, V4.z, V4.w};
-            keep = fh_tile_may_touch(a1, a2, a4, C.w, tx0, tx1, ty0, ty1);
+            keep = fh_ellipse_may_touch(er, rcp_rn(er[2]), rcp_rn(er[3]), tx0, tx1, ty0, ty1);
         }
         const uint64_t km = __ballot(keep);
         if (km == 0ull) continue;
@@ -234,20 +242,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 5 :
             sV2[pos] = V2;
             sV4[pos] = V4;
             sC[pos] = C;
-        }
-        __syncthreads();
-        // each staging lane adds its Gaussian's first-pass data, from its own
-        // LDS copy (the record's registers are free by now)
-        if (keep) {
-            const float4 W1 = sV1[pos], W2 = sV2[pos], W4 = sV4[pos];
-            const float o = sC[pos].w;
-            const float a1[4] = {W1.x, W1.y, W1.z, W1.w}, a2[4] = {W2.x, W2.y, W2.z, W2.w};
-            const float a4[4] = {W4.x, W4.y, W4.z, W4.w};
+            // first-pass data: the pixel conic about the tile centre, or
+            // (w = 1) no usable conic: the exact test against the cutoff bound
             float kq[6];
-            fh_pixel_conic(a1, a2, a4, o, ox, oy, kq);
+            fh_ellipse_pixel_conic(er, ox, oy, kq);
+            const bool exact = kq[5] < 0.0f;
+            float cut = 0.0f;
+            if (__ballot(exact) != 0ull) cut = exact ? fh_pixel_cutoff(C.w) : 0.0f;  // rare
             sQ1[pos] = make_float4(kq[0], kq[1], kq[2], kq[3]);
-            // w = 1: no usable conic, the exact test decides
-            sQ2[pos] = make_float4(kq[4], kq[5], fh_pixel_cutoff(o), kq[5] < 0.0f ? 1.0f : 0.0f);
+            sQ2[pos] = make_float4(kq[4], kq[5], cut, exact ? 1.0f : 0.0f);
         }
         if ((int)li >= count) {
             // a slot past the group's Gaussians (any lane there, staging or
@@ -520,6 +523,8 @@ hipError_t launch_blend(const ViewParams* dvp, const ViewBuffers& vb, const uint
     a.ranges = vb.ranges;
     a.vals = sorted_vals;
     a.prim = vb.prim;
+    a.ell_a = vb.ell_a;
+    a.ell_b = vb.ell_b;
     a.xcd_run = (uint32_t)(xcd_run > 0 ? xcd_run : 0);
     // rasterization/src/inference.cu:17-31: largest power of two <= K in 1..32
     if (K >= 32) blend_kernel<32><<<n_active, 64, 0, s>>>(dvp, a);

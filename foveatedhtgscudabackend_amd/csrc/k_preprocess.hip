@@ -3,12 +3,16 @@
 // rows VPMT1/2/4 of VPM*T, the exact screen-space extent of the quadric at
 // the opacity cutoff, tile bounds, tiles-touched from the two summed-area
 // tables, and the degree-0..3 SH colour. Also writes the per-block sum of
-// tiles-touched, which the instance kernel turns into write offsets.
+// tiles-touched, which the instance kernel turns into write offsets, and
+// each visible Gaussian's support-ellipse record (fh_ellipse_record), which
+// the blend's tile cull and first pass read instead of deriving it again for
+// every tile the Gaussian is listed in.
 //
 // Follows rasterization/include/kernels/inference.cuh:42-136,
 // rasterization/include/utils/kernel_utils.cuh:76-213 and
 // rasterization/include/utils/sh_utils.cuh:32-69.
 #include "fhtgs_device.h"
+#include "fhtgs_cull.h"
 
 namespace fhtgs {
 
@@ -107,6 +111,26 @@ __device__ __forceinline__ v3 sh_color(const float* sh0, const float* shr, v3 mu
 // Row `row` of VPM times [u v w | mu] (kernel_utils.cuh:135,144,152,156).
 __device__ __forceinline__ v4 frame_row(v4 row, v3 u, v3 v, v3 w, v3 mu) {
     return v4{dot3(row, u), dot3(row, v), dot3(row, w), dot3(row, mu) + row.w};
+}
+
+// the view's record row and support-ellipse record of a visible Gaussian
+// (both ellipse arrays are written with unit stride across the lanes). The
+// logarithm is taken again here rather than kept in a register through the
+// SH gathers, which need every register they can get.
+__device__ __forceinline__ void store_record(const ViewBuffers& vb, uint32_t i, v4 V1, v4 V2, v4 V4,
+                                             float4 rgba) {
+    const float c2 = 2.0f * fh_logf(rgba.w * 255.0f);
+    float4* rec = vb.prim + 4 * (size_t)i;
+    rec[0] = st4(V1);
+    rec[1] = st4(V2);
+    rec[2] = st4(V4);
+    rec[3] = rgba;
+    const float a1[4] = {V1.x, V1.y, V1.z, V1.w}, a2[4] = {V2.x, V2.y, V2.z, V2.w};
+    const float a4[4] = {V4.x, V4.y, V4.z, V4.w};
+    float r[5];
+    fh_ellipse_record(a1, a2, a4, c2, r);
+    vb.ell_a[i] = make_float4(r[0], r[1], r[2], r[3]);
+    vb.ell_b[i] = r[4];
 }
 
 __device__ __forceinline__ uint32_t tile_lo(float c, float e, int grid) {
@@ -334,11 +358,7 @@ __global__ __launch_bounds__(kPrimBlock) void preprocess_aaa_kernel(PreprocessAr
             const v3 cam{vp.cam[0], vp.cam[1], vp.cam[2]};
             const v3 rgb = sh_color(a.sh0 + 3 * (size_t)i, a.shr + (size_t)i * vp.total_sh * 3, mu, cam,
                                     vp.active_sh, vp.total_sh);
-            float4* rec = vb.prim + 4 * (size_t)i;
-            rec[0] = st4(V1);
-            rec[1] = st4(V2);
-            rec[2] = st4(V4);
-            rec[3] = make_float4(rgb.x, rgb.y, rgb.z, opacity);
+            store_record(vb, i, V1, V2, V4, make_float4(rgb.x, rgb.y, rgb.z, opacity));
         }
     }
     const uint32_t lane = lane_id(), wid = threadIdx.x >> 6;
@@ -517,11 +537,7 @@ __global__ __launch_bounds__(kPrimBlock) void preprocess_kernel(PreprocessArgs g
         if (!visible[v]) continue;
         const ViewBuffers& vb = a.vb[v];
         vb.bounds[i] = bnd[v];
-        float4* rec = vb.prim + 4 * (size_t)i;
-        rec[0] = st4(R1[v]);
-        rec[1] = st4(R2[v]);
-        rec[2] = st4(R4[v]);
-        rec[3] = col[v];
+        store_record(vb, i, R1[v], R2[v], R4[v], col[v]);
     }
 
 #pragma unroll

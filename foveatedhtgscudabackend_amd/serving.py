@@ -150,12 +150,13 @@ def make_requests(headsets: list, duration_s: float, width: int, height: int, fo
 # ---------------------------------------------------------------- server
 
 def slot_bytes(n_gaussians: int, width: int, height: int) -> int:
-    """Device bytes one eye's slot holds: per-Gaussian scratch (80 B) plus
-    tile tables and instance buffers for about 2.5 instances per Gaussian
-    with headroom."""
+    """Device bytes one eye's slot holds: per-Gaussian scratch (100 B: tiles
+    touched, bounds, the 64-B record row, the 20-B ellipse record, block
+    sums) plus tile tables and instance buffers for about 2.5 instances per
+    Gaussian with headroom."""
     gw, gh = (width + 15) // 16, (height + 15) // 16
     inst = int(2.5 * n_gaussians * 1.25) + 65536
-    return 80 * n_gaussians + 40 * gw * gh * 4 + inst * 16 + (4 << 20)
+    return 100 * n_gaussians + 40 * gw * gh * 4 + inst * 16 + (4 << 20)
 
 
 class Server:

@@ -1,10 +1,11 @@
-// Audit of the blend's per-tile cull (fh_tile_may_touch): over every (tile,
+// Audit of the blend's per-tile cull (fh_ellipse_may_touch on the ellipse
+// record fh_ellipse_record that the preprocess stores): over every (tile,
 // instance) pair of a frame, a Gaussian that the cull drops must reach no
 // sample of the tile, evaluated with the blend's own float32 operations
 // (cutoff, exp, alpha >= 1/255). Then, per pixel, the blend's first-pass
-// pre-test (fh_pixel_conic / fh_pixel_in_conic, or the exact num/den test
-// against fh_pixel_cutoff for Gaussians without a usable conic) must pass at
-// every sample a kept Gaussian reaches. Input is a binary dump of one oracle
+// pre-test (fh_ellipse_pixel_conic / fh_pixel_in_conic, or the exact num/den
+// test against the record's cutoff for Gaussians without a usable conic)
+// must pass at every sample a kept Gaussian reaches. Input is a binary dump of one oracle
 // frame written by tests/test_cull.py; prints pair counts and the number of
 // violations of each (both must be 0), and the smallest relative slack
 // (det - q) / det of the conic test at a reached sample.
@@ -96,7 +97,9 @@ int main(int argc, char** argv) {
                 const uint32_t p = vals[j];
                 const float *a1 = &v1[4 * (size_t)p], *a2 = &v2[4 * (size_t)p], *a4 = &v4[4 * (size_t)p];
                 const float o = rgba[4 * (size_t)p + 3];
-                const bool keep = fh_tile_may_touch(a1, a2, a4, o, x0, x1, y0, y1);
+                float rec[5];
+                fh_ellipse_record(a1, a2, a4, 2.0f * fh_logf(o * 255.0f), rec);
+                const bool keep = fh_ellipse_may_touch(rec, 1.0f / rec[2], 1.0f / rec[3], x0, x1, y0, y1);
                 bool any = false;
                 for (int q = 0; q < side * side && !any; ++q) {
                     const int px = bx + (q % side) * step, py = by + (q / side) * step;
@@ -108,7 +111,7 @@ int main(int argc, char** argv) {
                 if (keep) {
                     const float ox = 0.5f * (x0 + x1), oy = 0.5f * (y0 + y1);
                     float k[6];
-                    fh_pixel_conic(a1, a2, a4, o, ox, oy, k);
+                    fh_ellipse_pixel_conic(rec, ox, oy, k);
                     no_conic += k[5] < 0.0f;
                     const float cut = fh_pixel_cutoff(o);
                     for (int q = 0; q < side * side; ++q) {
