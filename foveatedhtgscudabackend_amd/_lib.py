@@ -44,7 +44,7 @@ STAGE_NAMES = ("clear", "tiles", "preprocess", "scan", "sync", "instances", "sor
 
 EXPORTED_SYMBOLS = (
     "fhtgs_create", "fhtgs_destroy", "fhtgs_last_error", "fhtgs_version", "fhtgs_render",
-    "fhtgs_workspace_bytes", "fhtgs_sort_pairs", "fhtgs_fovea_radius_tiles",
+    "fhtgs_workspace_bytes", "fhtgs_workspace_query", "fhtgs_sort_pairs", "fhtgs_fovea_radius_tiles",
     "fhtgs_debug_read", "fhtgs_debug_counts", "fhtgs_set_option", "fhtgs_stage_times",
     "fhtgs_render_batch", "fhtgs_autotune", "fhtgs_tuned_digit_bits", "fhtgs_get_option",
     "fhtgs_visualize_gaze", "fhtgs_visualize_tile_boundaries", "fhtgs_build_tables",
@@ -132,6 +132,8 @@ def load() -> ctypes.CDLL:
         lib.fhtgs_build_tables.restype = ctypes.c_int
         lib.fhtgs_workspace_bytes.argtypes = [vp, ctypes.POINTER(i64)]
         lib.fhtgs_workspace_bytes.restype = ctypes.c_int
+        lib.fhtgs_workspace_query.argtypes = [ctypes.POINTER(View), i64, i64, ctypes.POINTER(i64), ctypes.POINTER(i64)]
+        lib.fhtgs_workspace_query.restype = ctypes.c_int
         lib.fhtgs_sort_pairs.argtypes = [vp, vp, vp, i64, i32, i32, vp]
         lib.fhtgs_sort_pairs.restype = ctypes.c_int
         lib.fhtgs_fovea_radius_tiles.argtypes = [i32, i32, i32, i32, ctypes.POINTER(i32), ctypes.POINTER(i32)]
@@ -172,6 +174,20 @@ def validate_view(view: View) -> tuple[int, str]:
     msg = ctypes.create_string_buffer(256)
     rc = lib.fhtgs_validate_view(ctypes.byref(view), msg, 256)
     return rc, msg.value.decode()
+
+
+def workspace_query(view: View, n_primitives: int, n_instances: int) -> tuple[int, int]:
+    """(fixed, instance) device bytes one view slot needs for `view` over
+    n_primitives Gaussians with room for n_instances tile instances, before
+    any render (host only, no device call); raises ValueError for a view the
+    library rejects."""
+    lib = load()
+    f, i = ctypes.c_int64(0), ctypes.c_int64(0)
+    rc = lib.fhtgs_workspace_query(ctypes.byref(view), int(n_primitives), int(n_instances),
+                                   ctypes.byref(f), ctypes.byref(i))
+    if rc != FHTGS_OK:
+        raise ValueError(f"fhtgs_workspace_query: {lib.fhtgs_last_error(None).decode()} (status {rc})")
+    return f.value, i.value
 
 
 def sort_pairs_noalloc(keys_ptr: int, vals_ptr: int, n: int, key_bytes: int, end_bit: int,

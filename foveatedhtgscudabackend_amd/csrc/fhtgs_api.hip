@@ -264,11 +264,27 @@ void carve_inst(void* base, int64_t I, const SortPlan& plan, InstanceBuffers& ib
     bytes = align_up(c.off);
 }
 
-int validate_view(fhtgs_context* h, const fhtgs_view& v, int idx) {
-    const std::string p = "view " + std::to_string(idx) + ": ";
+// The checks that depend on the view's sizes and radii only.
+int validate_geometry(fhtgs_context* h, const fhtgs_view& v, const std::string& p) {
     if (v.width <= 0 || v.height <= 0) return set_err(h, FHTGS_E_INVALID, p + "width and height must be positive");
     if ((int64_t)v.width * v.height * 3 > (int64_t)1 << 31)
         return set_err(h, FHTGS_E_INVALID, p + "image too large");
+    if (v.foveation_radius_px >= 0 && v.blend_width_px > v.foveation_radius_px)
+        return set_err(h, FHTGS_E_INVALID, p + "blend width exceeds the foveation radius");
+    // the ring's cross-fade divides by blend_width_px - 16 (interpolation.cuh:244-250)
+    if (v.foveation_radius_px >= 0 && v.blend_width_px <= kTileL)
+        return set_err(h, FHTGS_E_INVALID, p + "blend width must exceed 16 px with foveation on");
+    const int64_t n_large = (int64_t)((v.width + kTileL - 1) / kTileL) * ((v.height + kTileL - 1) / kTileL);
+    if (n_large > kMaxLargeTiles)
+        return set_err(h, FHTGS_E_UNSUPPORTED, p + "more than 65536 16-px tiles (above 4096 x 4096 pixels)");
+    if (v.quarter_radius_px > 0 && (v.foveation_radius_px < 0 || v.quarter_radius_px < v.foveation_radius_px))
+        return set_err(h, FHTGS_E_INVALID, p + "the quarter radius needs foveation and must be >= its radius");
+    return FHTGS_OK;
+}
+
+int validate_view(fhtgs_context* h, const fhtgs_view& v, int idx) {
+    const std::string p = "view " + std::to_string(idx) + ": ";
+    if (const int rc = validate_geometry(h, v, p)) return rc;
     if (!v.M || !v.VPM || !v.cam_position || !v.image)
         return set_err(h, FHTGS_E_INVALID, p + "null camera or image pointer");
     // both tables given, or neither (built on the device each frame)
@@ -285,20 +301,8 @@ int validate_view(fhtgs_context* h, const fhtgs_view& v, int idx) {
     if ((v.background_model_type == 2 || v.background_model_type == 3) &&
         (v.background_width <= 0 || v.background_height <= 0))
         return set_err(h, FHTGS_E_INVALID, p + "texture background needs its size");
-    if (v.foveation_radius_px >= 0 && v.blend_width_px > v.foveation_radius_px)
-        return set_err(h, FHTGS_E_INVALID, p + "blend width exceeds the foveation radius");
-    // the ring's cross-fade divides by blend_width_px - 16 (interpolation.cuh:244-250)
-    if (v.foveation_radius_px >= 0 && v.blend_width_px <= kTileL)
-        return set_err(h, FHTGS_E_INVALID, p + "blend width must exceed 16 px with foveation on");
-    const int64_t n_large = (int64_t)((v.width + kTileL - 1) / kTileL) * ((v.height + kTileL - 1) / kTileL);
-    if (n_large > kMaxLargeTiles)
-        return set_err(h, FHTGS_E_UNSUPPORTED, p + "more than 65536 16-px tiles (above 4096 x 4096 pixels)");
-    if (v.quarter_radius_px > 0) {
-        if (v.foveation_radius_px < 0 || v.quarter_radius_px < v.foveation_radius_px)
-            return set_err(h, FHTGS_E_INVALID, p + "the quarter radius needs foveation and must be >= its radius");
-        if (v.blur_periphery)
-            return set_err(h, FHTGS_E_UNSUPPORTED, p + "the quarter level does not combine with blur_periphery");
-    }
+    if (v.quarter_radius_px > 0 && v.blur_periphery)
+        return set_err(h, FHTGS_E_UNSUPPORTED, p + "the quarter level does not combine with blur_periphery");
     return FHTGS_OK;
 }
 
@@ -937,6 +941,36 @@ int fhtgs_workspace_bytes(fhtgs_handle h, int64_t* bytes) {
     int64_t t = (int64_t)(h->sort_scratch.cap + h->params.cap);
     for (auto& s : h->slots) t += (int64_t)(s.fixed.cap + s.inst.cap + s.debug.cap);
     *bytes = t;
+    return FHTGS_OK;
+}
+
+int fhtgs_workspace_query(const fhtgs_view* view, int64_t n_primitives, int64_t n_instances,
+                          int64_t* fixed_bytes, int64_t* instance_bytes) {
+    if (!view || !fixed_bytes || !instance_bytes || n_primitives < 0 || n_instances < 0)
+        return set_err(nullptr, FHTGS_E_INVALID, "null view or output, or a negative count");
+    fhtgs_context probe;  // host-side only: collects the message, touches no device
+    if (const int rc = validate_geometry(&probe, *view, "view: ")) return set_err(nullptr, rc, probe.err);
+    fhtgs_scene sc{};
+    sc.n_primitives = n_primitives;
+    ViewParams vp{};
+    fill_params(sc, *view, vp);
+    // the same carving a render does: per-Gaussian and per-tile buffers (with
+    // device-built culling tables when the view brings none), then the
+    // instance buffers under the largest sort plan any digit width gives
+    const int tR = view->fovea_mask_area_table == nullptr ? vp.fovea_r : -1;
+    ViewBuffers vb{};
+    size_t fixed = 0;
+    carve_fixed(nullptr, n_primitives, vp.n_large, vp.n_small, vb, fixed, vp.gw, vp.gh, tR);
+    const int end_bit = end_bit_of((uint32_t)vp.n_small + 1u);
+    size_t inst = 0;
+    for (int bits = 4; bits <= 11; ++bits) {
+        InstanceBuffers ib{};
+        size_t b = 0;
+        carve_inst(nullptr, n_instances, make_plan(end_bit, bits), ib, b);
+        inst = std::max(inst, b);
+    }
+    *fixed_bytes = (int64_t)fixed;
+    *instance_bytes = (int64_t)inst;
     return FHTGS_OK;
 }
 

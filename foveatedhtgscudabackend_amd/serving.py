@@ -149,14 +149,18 @@ def make_requests(headsets: list, duration_s: float, width: int, height: int, fo
 
 # ---------------------------------------------------------------- server
 
-def slot_bytes(n_gaussians: int, width: int, height: int) -> int:
-    """Device bytes one eye's slot holds: per-Gaussian scratch (100 B: tiles
-    touched, bounds, the 64-B record row, the 20-B ellipse record, block
-    sums) plus tile tables and instance buffers for about 2.5 instances per
-    Gaussian with headroom."""
-    gw, gh = (width + 15) // 16, (height + 15) // 16
+def slot_bytes(n_gaussians: int, width: int, height: int, foveation: tuple = REFERENCE_FOVEATION) -> int:
+    """Device bytes one eye's slot holds, from the library's own sizing
+    (fhtgs_workspace_query): the per-Gaussian and per-tile buffers of a view
+    of this size with device-built culling tables, plus instance buffers for
+    about 2.5 instances per Gaussian with the headroom a device-count frame
+    adds (25 % + 65536)."""
+    fr, bw, qr = (int(x) for x in foveation)
+    view = _lib.View(width=int(width), height=int(height), K=8, foveation_radius_px=fr,
+                     blend_width_px=bw, quarter_radius_px=qr)
     inst = int(2.5 * n_gaussians * 1.25) + 65536
-    return 100 * n_gaussians + 40 * gw * gh * 4 + inst * 16 + (4 << 20)
+    fixed, inst_bytes = _lib.workspace_query(view, n_gaussians, inst)
+    return fixed + inst_bytes
 
 
 class Server:
@@ -188,7 +192,8 @@ class Server:
         """Admits a headset if the device can hold its slots (in every
         worker's handle) beside what is already allocated."""
         free, _ = torch.cuda.mem_get_info(self.device)
-        need = 2 * len(self.handles) * slot_bytes(int(self.scenes[scene][0].shape[0]), width, height)
+        need = 2 * len(self.handles) * slot_bytes(int(self.scenes[scene][0].shape[0]), width, height,
+                                                  self.foveation)
         if free - need < self.reserve:
             self.rejected.append(hid)
             return False

@@ -30,6 +30,9 @@
  *   fhtgs_visualize_tile_boundaries <- visualize_tile_boundaries_wrapper,
  *                                visualization_api.h:16-20, visualization_api.cu:37-63
  *                                (_C_visualization.visualize_tile_boundaries, bindings_visualization.cpp:9)
+ *   fhtgs_workspace_query     <- required<T>(P, ...) over the from_blob layouts
+ *                                (rasterization/include/utils/buffer_utils.h:17-22,
+ *                                36-143): scratch bytes from element counts
  *   fhtgs_workspace_bytes     <- the per-call scratch sizing of required<>/from_blob,
  *                                rasterization/include/utils/buffer_utils.h:10-143
  *   fhtgs_sort_pairs          <- cub::DeviceRadixSort::SortPairs over bits [0,end_bit),
@@ -183,6 +186,24 @@ int fhtgs_autotune(fhtgs_handle h, const fhtgs_scene* scenes, int32_t n_scenes,
 /* The digit width the handle's sort uses for an image size (tuned or the
  * FHTGS_OPT_SORT_DIGIT_BITS default). */
 int fhtgs_tuned_digit_bits(fhtgs_handle h, int32_t width, int32_t height, int32_t* bits);
+
+/* Device bytes one view slot needs, before any render (no handle, no
+ * device call): *fixed_bytes for the per-Gaussian and per-tile buffers of
+ * `view` over a scene of n_primitives Gaussians (with device-built culling
+ * tables when the view brings no summed-area tables), *instance_bytes for
+ * n_instances tile instances (keys, values and the sort's counters, under
+ * the largest plan any digit width gives). A handle holds one slot per view
+ * it renders -- per eye for stereo -- which only grows, plus a few KB of
+ * parameters. The instance count of a frame is known after its preprocess;
+ * frames rendered without a mid-frame wait (device-count and graph modes)
+ * size their slot for the last exact count + 25 % + 65536. Reads only the
+ * view's sizes, foveation radii and whether it brings summed-area tables
+ * (its pointers may be NULL), and checks those as fhtgs_validate_view does
+ * (the reason, on failure, from fhtgs_last_error(NULL)). The reference computes the same kind of figure
+ * per call from element counts (required<T>, utils/buffer_utils.h:17-22) and
+ * allocates it inside every call. */
+int fhtgs_workspace_query(const fhtgs_view* view, int64_t n_primitives, int64_t n_instances,
+                          int64_t* fixed_bytes, int64_t* instance_bytes);
 
 /* Device scratch the handle currently holds (it grows on demand). */
 int fhtgs_workspace_bytes(fhtgs_handle h, int64_t* bytes);

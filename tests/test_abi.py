@@ -123,3 +123,35 @@ def test_view_checks_run_on_the_host_before_any_enqueue(kw, code, why):
     rc, msg = _lib.validate_view(_valid_view(**kw))
     assert rc == code, msg
     assert why in msg
+
+
+def test_workspace_query_sizes_a_slot_from_counts_on_the_host():
+    """fhtgs_workspace_query gives a slot's bytes from the view's geometry and
+    the element counts, before anything exists on the device (pointers may be
+    NULL): the per-Gaussian part grows by the buffers the preprocess writes
+    (tiles touched 4, bounds 8, record row 64, ellipse record 20, block sums
+    4/256 B), the instance part by double-buffered 32-bit keys and values."""
+    from foveatedhtgscudabackend_amd import _lib
+    geo = dict(K=8, width=2064, height=2208, foveation_radius_px=232, blend_width_px=77, quarter_radius_px=606)
+    v = _lib.View(**geo)
+    f0, i0 = _lib.workspace_query(v, 0, 0)
+    f1, i1 = _lib.workspace_query(v, 3_000_000, 4_800_000)
+    assert 0 < f0 < f1 and 0 < i0 < i1
+    per_gauss = (f1 - f0) / 3_000_000
+    assert 4 + 8 + 64 + 20 <= per_gauss <= 4 + 8 + 64 + 20 + 1, per_gauss
+    per_inst = (i1 - i0) / 4_800_000
+    assert 16 <= per_inst <= 17, per_inst  # 2 x (4-B key + 4-B value), plus look-back status
+    # the tile part follows the image size; tables brought by the caller
+    # need no device-built copies
+    f_small, _ = _lib.workspace_query(_lib.View(**dict(geo, width=640, height=480, quarter_radius_px=0)), 0, 0)
+    assert f_small < f0
+    f_tabs, _ = _lib.workspace_query(_lib.View(**geo, visibility_mask=8, visibility_mask_area_table=8,
+                                               fovea_mask_area_table=8), 0, 0)
+    assert f_tabs < f0
+    # geometry the library rejects is rejected here too
+    with pytest.raises(ValueError, match="65536"):
+        _lib.workspace_query(_lib.View(**dict(geo, width=8193, height=8192)), 10, 10)
+    with pytest.raises(ValueError, match="blend width"):
+        _lib.workspace_query(_lib.View(**dict(geo, blend_width_px=16)), 10, 10)
+    with pytest.raises(ValueError):
+        _lib.workspace_query(v, -1, 0)

@@ -300,3 +300,32 @@ def test_repeated_frames_are_bit_identical(restore_mode):
         torch.cuda.synchronize()
         for eye, (g, e) in enumerate(zip(got, ref)):
             _same(g.cpu().numpy(), e, f"replay {k} eye {eye}")
+
+
+def test_workspace_query_matches_what_a_render_allocates():
+    """A fresh handle that renders one view in exact mode holds that view's
+    slot -- the query's per-Gaussian and per-tile bytes and instance buffers
+    for the frame's own instance count -- plus a small parameter block."""
+    from foveatedhtgscudabackend_amd import _binding, _lib
+    from foveatedhtgscudabackend_amd import scene as S
+    case = Case("workspace", 50000, 800, 600, seed=63, foveation_radius_px=200, blend_width_px=60)
+    sc = S.make_scene(case.n, seed=case.seed, cube=case.cube)
+    args = S.scene_to_torch(sc, "cuda")
+    cam = S.orbit_camera(case.width, case.height, case.fov_deg, seed=1063)
+    st = S.settings_for(cam, S.seeded_gaze(cam, 2063))
+    h = _lib.Handle(0)
+    h.set_option(_lib.OPT_FRAME_MODE, _lib.FRAME_EXACT)
+    keep: list = []
+    scene, device = _binding.scene_struct(*args, keep=keep)
+    view, img = _binding.view_struct(device, *st.as_tuple(), False, False, False, 200, 60, False, keep=keep)
+    stream = torch.cuda.current_stream()
+    h.render_batch([scene], [view], [0], stream.cuda_stream, [0])
+    torch.cuda.synchronize()
+    n_inst = h.counts(0)[4]
+    assert n_inst > 10000
+    fixed, inst = _lib.workspace_query(view, case.n, n_inst)
+    held = h.workspace_bytes()
+    # 7600 small tiles: 16-bit keys, so 2 x (2 + 4) B per instance at least
+    assert fixed + 12 * n_inst <= held <= fixed + inst + 65536, (held, fixed, inst, n_inst)
+    # a frame with fewer instances fits the same query figure
+    assert _lib.workspace_query(view, case.n, n_inst // 2)[1] < inst
