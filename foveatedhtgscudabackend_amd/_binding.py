@@ -98,11 +98,16 @@ def view_struct(device, M, VPM, VPR_inv, cam_position, gaze_position, visibility
     if visibility_mask is not None and visibility_mask.numel() < (gw * gh + 31) // 32:
         raise ValueError("visibility_mask is smaller than ceil(tiles/32) words")
     if not auto:
-        if visibility_mask_area_table.numel() < (gw + 1) * (gh + 1):
-            raise ValueError("visibility_mask_area_table is smaller than (gw+1)*(gh+1)")
+        # summed-area tables are laid out for one image size and one foveation
+        # radius: a table of another size would be read with the wrong stride
+        if visibility_mask_area_table.numel() != (gw + 1) * (gh + 1):
+            raise ValueError(f"visibility_mask_area_table must hold (gw+1)*(gh+1) = {(gw + 1) * (gh + 1)} "
+                             f"entries for {width}x{height}, got {visibility_mask_area_table.numel()}")
         R, _ = _lib.fovea_radius_tiles(width, height, int(foveation_radius_px), int(blend_width_px))
-        if fovea_mask_area_table.numel() < (2 * R + 1) ** 2:
-            raise ValueError(f"fovea_mask_area_table must hold (2*{R}+1)^2 entries")
+        if fovea_mask_area_table.numel() != (2 * R + 1) ** 2:
+            raise ValueError(f"fovea_mask_area_table must hold (2*{R}+1)^2 = {(2 * R + 1) ** 2} entries for "
+                             f"this foveation radius, got {fovea_mask_area_table.numel()} (tables built for "
+                             "another radius?)")
     for field, t in (("visibility_mask", visibility_mask),
                      ("visibility_mask_area_table", visibility_mask_area_table),
                      ("fovea_mask_area_table", fovea_mask_area_table)):

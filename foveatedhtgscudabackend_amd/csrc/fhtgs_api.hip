@@ -169,6 +169,10 @@ std::mutex& device_memory_lock() {
 // the message of the last failure of a call without a handle, per thread
 thread_local std::string g_no_handle_err;
 
+const char* const kTablesMismatch =
+    "the culling tables do not fit the view (a summed-area table of another image size or "
+    "foveation radius): tiles-touched exceeds 4 per tile of a Gaussian's bounds";
+
 int set_err(fhtgs_context* h, int code, const std::string& msg) {
     if (h) h->err = msg;
     else g_no_handle_err = msg;
@@ -706,6 +710,7 @@ int render_impl(fhtgs_context* h, const fhtgs_scene* scenes, int n_scenes, const
         HCHK(hipStreamSynchronize(s));
         for (int v = 0; v < n_views; ++v) {
             ViewSlot& slot = (*h->cur[v]);
+            if (slot.host_fi[FI_ERROR] & 8u) return set_err(h, FHTGS_E_TABLES, kTablesMismatch);
             const uint32_t I = slot.n_prims > 0 ? slot.host_fi[FI_INSTANCES] : 0u;
             if (I >= (1u << 30)) return set_err(h, FHTGS_E_INVALID, "more than 2^30 tile instances in one view");
             slot.n_instances = I;
@@ -776,6 +781,7 @@ int render_impl(fhtgs_context* h, const fhtgs_scene* scenes, int n_scenes, const
     for (int v = 0; v < n_views; ++v) {
         ViewSlot& slot = (*h->cur[v]);
         const uint32_t* fi = slot.host_fi;
+        if (fi[FI_ERROR] & 8u) return set_err(h, FHTGS_E_TABLES, kTablesMismatch);
         if (fi[FI_ERROR] & 1u)
             return set_err(h, FHTGS_E_TABLES,
                            "tiles-touched from the culling tables under-counts the visible tiles of a Gaussian");

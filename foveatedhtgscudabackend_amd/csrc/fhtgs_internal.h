@@ -57,7 +57,8 @@ enum FrameInfo : int {
     FI_INSTANCES = 9,     // I = sum of tiles-touched
     FI_VALID = 10,        // non-sentinel instances (atomic)
     FI_ERROR = 11,        // bit 0: tiles-touched under-count, bit 1: sort inconsistency,
-                          // bit 2: instance capacity exceeded
+                          // bit 2: instance capacity exceeded, bit 3: culling tables that
+                          // do not fit the view (impossible tiles-touched)
     FI_N_VISIBLE = 12,    // preprocess survivors (atomic)
     FI_WORDS = 16
 };

@@ -161,8 +161,10 @@ __global__ __launch_bounds__(kScanThreads) void scan_blocks_kernel(uint32_t* sum
                                                                    uint32_t* frame_info) {
     __shared__ uint32_t s_v[kScanChunk];
     __shared__ uint32_t s_w[kScanThreads / 64];
+    __shared__ unsigned long long s_w64[kScanThreads / 64];
     const uint32_t tid = threadIdx.x, lane = lane_id(), wid = tid >> 6;
     uint32_t carry = 0;
+    unsigned long long mine = 0;  // this thread's share of the total, without wrap
     for (uint32_t c0 = 0; c0 < nb; c0 += kScanChunk) {
         uint32_t x[kScanPer];
 #pragma unroll
@@ -178,6 +180,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_blocks_kernel(uint32_t* sum
         for (uint32_t k = 0; k < kScanPer; ++k) {
             x[k] = s_v[tid * kScanPer + k];
             local += x[k];
+            mine += x[k];
         }
         const uint32_t incl = wave_incl_scan(local);
         if (lane == 63) s_w[wid] = incl;
@@ -202,7 +205,17 @@ __global__ __launch_bounds__(kScanThreads) void scan_blocks_kernel(uint32_t* sum
         carry += tot;
         __syncthreads();  // s_v and s_w are reused by the next chunk
     }
-    if (tid == 0) frame_info[FI_INSTANCES] = carry;
+    // the 32-bit offsets hold only if the total does: a larger total
+    // (possible only with tiles-touched near its bound everywhere) is
+    // published as 0xffffffff, which every frame mode rejects
+    for (int o = 32; o > 0; o >>= 1) mine += __shfl_xor(mine, o, 64);
+    if (lane == 0) s_w64[wid] = mine;
+    __syncthreads();
+    if (tid == 0) {
+        unsigned long long total = 0;
+        for (uint32_t w = 0; w < kScanThreads / 64; ++w) total += s_w64[w];
+        frame_info[FI_INSTANCES] = total > 0xffffffffull ? 0xffffffffu : carry;
+    }
 }
 
 hipError_t launch_scan_blocks(const ViewBuffers& vb, uint32_t n_blocks, hipStream_t s) {

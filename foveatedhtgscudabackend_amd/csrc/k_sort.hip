@@ -70,7 +70,8 @@ __global__ __launch_bounds__(kPrimBlock) void instances_kernel(const ViewParams*
                                                               uint32_t n) {
     // a frame with more instances than the buffers hold writes nothing and
     // reports it; the host renders it again with exact buffers
-    if ((int64_t)vb.frame_info[FI_INSTANCES] > ib.capacity) {
+    const uint32_t n_inst = vb.frame_info[FI_INSTANCES];
+    if ((int64_t)n_inst > ib.capacity) {
         if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&vb.frame_info[FI_ERROR], 4u);
         return;
     }
@@ -227,10 +228,16 @@ __global__ __launch_bounds__(kPrimBlock) void instances_kernel(const ViewParams*
         const bool vis = nt0 != 0u;
         const uint64_t m = __ballot(vis);
         if (vis) {
+            // every write stays inside the Gaussian's range and the range
+            // inside the frame's I slots (the preprocess and the scan keep
+            // tiles-touched and its total in range; this holds the bound
+            // whatever they were given)
+            const bool fits = nt0 <= n_inst && start <= n_inst - nt0;
+            overflow |= !fits;
             const uint32_t pos = qn + (uint32_t)__popcll(m & lt);
             q[pos] = i;
-            q[kInstQueue + pos] = start;
-            q[2 * kInstQueue + pos] = start + nt0;
+            q[kInstQueue + pos] = fits ? start : 0u;
+            q[2 * kInstQueue + pos] = fits ? start + nt0 : 0u;
             q[3 * kInstQueue + pos] = (uint32_t)b0.x | ((uint32_t)b0.y << 16);
             q[4 * kInstQueue + pos] = (uint32_t)b0.z | ((uint32_t)b0.w << 16);
         }
