@@ -9,9 +9,19 @@
 // frame written by tests/test_cull.py; prints pair counts and the number of
 // violations of each (both must be 0), and the smallest relative slack
 // (det - q) / det of the conic test at a reached sample.
+//
+// With --waves it prints instead the blend's wave-level loop counts for the
+// frame, as the kernel runs it (batches of 64 staged Gaussians per 8x8 wave,
+// four 16-lane QUARTER groups of 16 each): staging rounds, first-pass steps
+// (the wave's largest kept count per round) and second-pass steps (the
+// largest number of Gaussians any lane of the wave evaluates per round),
+// beside what larger batches and a 16x16 work tile sharing one staging for
+// the four sub-tiles of a FOVEA or BLENDED tile would give.
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
+#include <string>
+#include <algorithm>
 #include "../foveatedhtgscudabackend_amd/csrc/fhtgs_cull.h"
 
 static float dot3(const float* a, const float* b) { return fmaf(a[2], b[2], fmaf(a[1], b[1], a[0] * b[0])); }
@@ -64,8 +74,151 @@ static bool rd(FILE* f, std::vector<T>& v, size_t n) {
     return n == 0 || fread(v.data(), sizeof(T), n, f) == n;
 }
 
+struct Frame {
+    int64_t gw = 0, W = 0, H = 0;
+    std::vector<uint32_t> part, tmp, ranges, vals;
+    std::vector<float> v1, v2, v4, rgba;
+};
+
+// The sample positions of partitioned tile tg (8x8, or 4x4 at stride 4 for
+// QUARTER), -1 outside the image, and their rectangle.
+static std::vector<std::pair<int, int>> samples(const Frame& F, int type, uint32_t tg, float r[4]) {
+    const int step = type == 3 ? 4 : (type == 1 ? 2 : 1), side = type == 3 ? 4 : 8;
+    const uint32_t lt = tg / 4, st = tg % 4;
+    const int bx = (int)(lt % F.gw) * 16 + (int)(st % 2) * 8, by = (int)(lt / F.gw) * 16 + (int)(st / 2) * 8;
+    r[0] = (float)bx; r[1] = (float)(bx + (side - 1) * step);
+    r[2] = (float)by; r[3] = (float)(by + (side - 1) * step);
+    std::vector<std::pair<int, int>> px;
+    for (int q = 0; q < side * side; ++q) {
+        const int x = bx + (q % side) * step, y = by + (q / side) * step;
+        px.push_back(x < F.W && y < F.H ? std::make_pair(x, y) : std::make_pair(-1, -1));
+    }
+    return px;
+}
+
+// One lane group walking list [b, e) in batches of bs, culled against rect
+// `cull`, first-pass tested at `px` about the rect's centre: per batch the
+// kept count and each sample's count of passing Gaussians.
+static void walk(const Frame& F, const std::vector<std::pair<int, int>>& px, const float cull[4], uint32_t b,
+                 uint32_t e, uint32_t bs, std::vector<int>& kept, std::vector<std::vector<int>>& lane) {
+    const float ox = 0.5f * (cull[0] + cull[1]), oy = 0.5f * (cull[2] + cull[3]);
+    for (uint32_t c = b; c < e; c = (e - c > bs) ? c + bs : e) {
+        int k = 0;
+        std::vector<int> cnt(px.size(), 0);
+        const uint32_t stop = (e - c > bs) ? c + bs : e;
+        for (uint32_t j = c; j < stop; ++j) {
+            const uint32_t p = F.vals[j];
+            const float *a1 = &F.v1[4 * (size_t)p], *a2 = &F.v2[4 * (size_t)p], *a4 = &F.v4[4 * (size_t)p];
+            const float o = F.rgba[4 * (size_t)p + 3];
+            float rec[5];
+            fh_ellipse_record(a1, a2, a4, 2.0f * fh_logf(o * 255.0f), rec);
+            if (!fh_ellipse_may_touch(rec, 1.0f / rec[2], 1.0f / rec[3], cull[0], cull[1], cull[2], cull[3]))
+                continue;
+            ++k;
+            float kq[6];
+            fh_ellipse_pixel_conic(rec, ox, oy, kq);
+            const float cut = fh_pixel_cutoff(o);
+            for (size_t q = 0; q < px.size(); ++q)
+                if (px[q].first >= 0)
+                    cnt[q] += pre_test(kq, cut, a1, a2, a4, (float)px[q].first, (float)px[q].second, ox, oy, nullptr);
+        }
+        kept.push_back(k);
+        lane.push_back(cnt);
+    }
+}
+
+static int max_of(const std::vector<int>& v) { return v.empty() ? 0 : *std::max_element(v.begin(), v.end()); }
+
+// second-pass steps when g consecutive batches are evaluated as one
+static double merged(const std::vector<std::vector<int>>& lane, size_t g) {
+    double s = 0;
+    for (size_t i = 0; i < lane.size(); i += g) {
+        std::vector<int> sum(lane.empty() ? 0 : lane[0].size(), 0);
+        for (size_t j = i; j < lane.size() && j < i + g; ++j)
+            for (size_t q = 0; q < sum.size(); ++q) sum[q] += lane[j][q];
+        s += max_of(sum);
+    }
+    return s;
+}
+
+static void wave_report(const Frame& F) {
+    const char* names[4] = {"FOVEA", "PERIPHERY", "BLENDED", "QUARTER"};
+    double tot[3] = {0, 0, 0};
+    for (int type = 0; type < 4; ++type) {
+        double rounds = 0, p1 = 0, p2 = 0, alt[3] = {0, 0, 0}, wt_rounds = 0, wt_p1 = 0, units = 0;
+        const uint32_t g0 = F.part[2 * type], g1 = F.part[2 * type + 1];
+        if (type == 3) {
+            // four tiles per wave, 16 lanes and 16 staged Gaussians each; the
+            // wave runs as many rounds and steps as its largest group needs
+            for (uint32_t g = g0; g < g1; g += 4) {
+                ++units;
+                for (int v = 0; v < 3; ++v) {
+                    const uint32_t bs = v == 0 ? 16u : (v == 1 ? 32u : 0xffffffffu);
+                    std::vector<std::vector<int>> kept(4);
+                    std::vector<std::vector<std::vector<int>>> lane(4);
+                    size_t nb = 0;
+                    for (uint32_t t = 0; t < 4 && g + t < g1; ++t) {
+                        float r[4];
+                        const uint32_t tg = F.tmp[g + t];
+                        const auto px = samples(F, type, tg, r);
+                        walk(F, px, r, F.ranges[2 * (size_t)tg], F.ranges[2 * (size_t)tg + 1], bs, kept[t], lane[t]);
+                        nb = std::max(nb, kept[t].size());
+                    }
+                    for (size_t i = 0; i < nb; ++i) {
+                        int km = 0, pm = 0;
+                        for (int t = 0; t < 4; ++t)
+                            if (i < kept[t].size()) { km = std::max(km, kept[t][i]); pm = std::max(pm, max_of(lane[t][i])); }
+                        if (v == 0) { rounds += 1; p1 += km; p2 += pm; }
+                        alt[v] += pm;
+                    }
+                }
+            }
+            std::printf("waves %-9s units %.0f rounds %.0f pass1 %.0f pass2 %.0f | pass2 with 32 per group %.0f, "
+                        "whole lists %.0f\n", names[type], units, rounds, p1, p2, alt[1], alt[2]);
+        } else {
+            for (uint32_t g = g0; g < g1; ++g) {
+                ++units;
+                float r[4];
+                const uint32_t tg = F.tmp[g];
+                const auto px = samples(F, type, tg, r);
+                std::vector<int> kept;
+                std::vector<std::vector<int>> lane;
+                walk(F, px, r, F.ranges[2 * (size_t)tg], F.ranges[2 * (size_t)tg + 1], 64, kept, lane);
+                rounds += kept.size();
+                for (size_t i = 0; i < kept.size(); ++i) { p1 += kept[i]; p2 += max_of(lane[i]); }
+                alt[0] += merged(lane, 2);
+                alt[1] += merged(lane, 4);
+                alt[2] += merged(lane, lane.size() + 1);
+                if (type != 1 && tg % 4 == 0) {
+                    // 16x16 work tile: the sub-tiles' lists are the same, staged
+                    // once, culled against the whole tile, each wave's first
+                    // pass over every kept Gaussian
+                    const uint32_t lt = tg / 4;
+                    const float big[4] = {(float)((lt % F.gw) * 16), (float)((lt % F.gw) * 16 + 15),
+                                          (float)((lt / F.gw) * 16), (float)((lt / F.gw) * 16 + 15)};
+                    for (uint32_t st = 0; st < 4; ++st) {
+                        float rs[4];
+                        const auto pxs = samples(F, type, lt * 4 + st, rs);
+                        std::vector<int> k2;
+                        std::vector<std::vector<int>> l2;
+                        walk(F, pxs, big, F.ranges[2 * (size_t)tg], F.ranges[2 * (size_t)tg + 1], 64, k2, l2);
+                        if (st == 0) wt_rounds += k2.size();
+                        for (int k : k2) wt_p1 += k;
+                    }
+                }
+            }
+            std::printf("waves %-9s units %.0f rounds %.0f pass1 %.0f pass2 %.0f | pass2 with batches of 128 %.0f, "
+                        "256 %.0f, whole lists %.0f", names[type], units, rounds, p1, p2, alt[0], alt[1], alt[2]);
+            if (type != 1) std::printf(" | 16x16 work tile: staging rounds %.0f, pass1 %.0f", wt_rounds, wt_p1);
+            std::printf("\n");
+        }
+        tot[0] += rounds; tot[1] += p1; tot[2] += p2;
+    }
+    std::printf("waves total rounds %.0f pass1 %.0f pass2 %.0f\n", tot[0], tot[1], tot[2]);
+}
+
 int main(int argc, char** argv) {
-    if (argc < 2) { std::fprintf(stderr, "usage: cull_audit dump.bin\n"); return 2; }
+    if (argc < 2) { std::fprintf(stderr, "usage: cull_audit dump.bin [--waves]\n"); return 2; }
     FILE* f = std::fopen(argv[1], "rb");
     if (!f) { std::perror("open"); return 2; }
     int64_t h[7];
@@ -79,6 +232,14 @@ int main(int argc, char** argv) {
         return 2;
     }
     std::fclose(f);
+    if (argc > 2 && std::string(argv[2]) == "--waves") {
+        Frame F;
+        F.gw = gw; F.W = W; F.H = H;
+        F.part = part; F.tmp = tmp; F.ranges = ranges; F.vals = vals;
+        F.v1 = v1; F.v2 = v2; F.v4 = v4; F.rgba = rgba;
+        wave_report(F);
+        return 0;
+    }
     int64_t pairs = 0, kept = 0, reach = 0, bad = 0;
     int64_t px_reach = 0, px_pre = 0, px_bad = 0, px_evals = 0, no_conic = 0;
     double min_slack = 1e30;
