@@ -300,18 +300,18 @@ __device__ bool aaa_transform_cull(const ViewParams& vp, const Frame& F, v3 mu, 
     return false;
 }
 
-// The most instances a Gaussian with tile bounds [b0, b1) x [b2, b3) can
-// have: kNumSub per tile (0 for empty bounds; no wrap-around).
-__device__ __forceinline__ uint64_t touched_bound(uint32_t b0, uint32_t b1, uint32_t b2, uint32_t b3) {
-    return (uint64_t)kNumSub * (uint64_t)(b1 > b0 ? b1 - b0 : 0u) * (uint64_t)(b3 > b2 ? b3 - b2 : 0u);
-}
+// The most instances any Gaussian can have: kNumSub per tile of the largest
+// grid. A summed-area table of another image size or radius can give a
+// rectangle sum below zero, which wraps to a value far above this; such a
+// value would wrap the instance offsets and is refused. (Smaller wrong
+// counts only under- or over-fill the Gaussian's own range, which the
+// instance kernel reports or fills with sentinels.)
+constexpr uint32_t kMaxTouched = (uint32_t)kNumSub * (uint32_t)kMaxLargeTiles;
+constexpr uint32_t kTouchedRefused = 0xffffffffu;
 
 // Tiles-touched from the visibility and fovea summed-area tables
 // (kernel_utils.cuh:168-210): at most kNumSub per tile of the bounds when
-// the tables fit the view. Tables of another size or radius can give any
-// value, a negative rectangle sum wrapping to a huge one; the callers treat
-// anything above kNumSub x (tiles in the bounds) as an error of the frame
-// instead of letting it into the instance offsets.
+// the tables fit the view; the callers refuse values above kMaxTouched.
 __device__ __forceinline__ uint32_t tiles_touched(const ViewParams& vp, uint32_t b0, uint32_t b1, uint32_t b2,
                                                   uint32_t b3) {
     const int fd = 2 * vp.fovea_r;
@@ -340,7 +340,6 @@ __global__ __launch_bounds__(kPrimBlock) void preprocess_aaa_kernel(PreprocessAr
     const bool valid = i < a.n;
     uint32_t touched[NV];
     bool visible[NV];
-    uint32_t bad_tables = 0u;  // bit v: view v's tables gave an impossible tiles-touched
 #pragma unroll
     for (int v = 0; v < NV; ++v) { touched[v] = 0u; visible[v] = false; }
     if (valid) {
@@ -362,8 +361,11 @@ __global__ __launch_bounds__(kPrimBlock) void preprocess_aaa_kernel(PreprocessAr
             v4 V1, V2, V4;
             if (aaa_transform_cull(vp, F, mu, os, opacity, b0, b1, b2, b3, V1, V2, V4)) continue;
             const uint32_t nt = tiles_touched(vp, b0, b1, b2, b3);
-            if (nt > touched_bound(b0, b1, b2, b3)) {  // tables that do not fit the view
-                bad_tables |= 1u << v;
+            if (nt > kMaxTouched) {
+                // tables that do not fit the view: marked in the word that is
+                // live anyway, reported and cleared at the end; the Gaussian
+                // reserves nothing
+                touched[v] = kTouchedRefused;
                 continue;
             }
             if (nt == 0u) continue;
@@ -380,11 +382,13 @@ __global__ __launch_bounds__(kPrimBlock) void preprocess_aaa_kernel(PreprocessAr
     const uint32_t lane = lane_id(), wid = threadIdx.x >> 6;
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
+        const bool refused = touched[v] == kTouchedRefused;
+        if (refused) touched[v] = 0u;
+        if (__ballot(refused) != 0ull && lane == 0) atomicOr(&a.vb[v].frame_info[FI_ERROR], 8u);
         if (valid) a.vb[v].n_touched[i] = touched[v];
         const uint32_t ws = wave_sum(touched[v]);
         const uint32_t wv = (uint32_t)__popcll(__ballot(visible[v]));
         if (lane == 0) { s_sum[v][wid] = ws; s_vis[v][wid] = wv; }
-        if (__ballot((bad_tables >> v) & 1u) != 0ull && lane == 0) atomicOr(&a.vb[v].frame_info[FI_ERROR], 8u);
     }
     __syncthreads();
     // thread v finishes view v (static indices keep the parameters in registers)
@@ -410,7 +414,6 @@ __global__ __launch_bounds__(kPrimBlock) void preprocess_kernel(PreprocessArgs g
 
     uint32_t touched[NV];
     bool visible[NV];
-    uint32_t bad_tables = 0u;  // bit v: view v's tables gave an impossible tiles-touched
     // per view: the record rows and bounds, stored after the SH loads (one
     // counter tracks loads and stores, so loads after stores wait for them)
     v4 R1[NV], R2[NV], R4[NV];
@@ -473,8 +476,11 @@ __global__ __launch_bounds__(kPrimBlock) void preprocess_kernel(PreprocessArgs g
                 const uint32_t b0 = tile_lo(cx, ex, vp.gw), b1 = tile_hi(cx, ex, vp.gw);
                 const uint32_t b2 = tile_lo(cy, ey, vp.gh), b3 = tile_hi(cy, ey, vp.gh);
                 const uint32_t nt = tiles_touched(vp, b0, b1, b2, b3);
-                if (nt > touched_bound(b0, b1, b2, b3)) {  // tables that do not fit the view
-                    bad_tables |= 1u << v;
+                if (nt > kMaxTouched) {
+                    // tables that do not fit the view: marked in the word that is
+                    // live anyway, reported and cleared at the end; the Gaussian
+                    // reserves nothing
+                    touched[v] = kTouchedRefused;
                     continue;
                 }
                 if (nt == 0u) continue;
@@ -564,11 +570,13 @@ __global__ __launch_bounds__(kPrimBlock) void preprocess_kernel(PreprocessArgs g
 
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
+        const bool refused = touched[v] == kTouchedRefused;
+        if (refused) touched[v] = 0u;
+        if (__ballot(refused) != 0ull && lane == 0) atomicOr(&a.vb[v].frame_info[FI_ERROR], 8u);
         if (valid) a.vb[v].n_touched[i] = touched[v];
         const uint32_t ws = wave_sum(touched[v]);
         const uint32_t wv = (uint32_t)__popcll(__ballot(visible[v]));
         if (lane == 0) { s_sum[v][wid] = ws; s_vis[v][wid] = wv; }
-        if (__ballot((bad_tables >> v) & 1u) != 0ull && lane == 0) atomicOr(&a.vb[v].frame_info[FI_ERROR], 8u);
     }
     __syncthreads();
     // thread v finishes view v (static indices keep the parameters in registers)
