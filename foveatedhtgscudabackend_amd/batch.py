@@ -75,10 +75,12 @@ def render_batch(scenes: list, views: list) -> list:
 
 def autotune(scenes: list, views: list, frames: int = 3) -> dict:
     """Tunes the handle for this batch (sort digit width per view size, blend
-    XCD run; images are unchanged by both) and returns the choices."""
+    XCD run, concurrent view branches; images are unchanged by all three) and
+    returns the choices."""
     handle, structs, vstructs, index, slots, images, stream, keep = _marshal(scenes, views)
     handle.autotune(structs, vstructs, index, stream, frames, slots)
     del keep
     sizes = sorted({(bv.settings.width, bv.settings.height) for bv in views})
     return {"digit_bits": {f"{w}x{h}": handle.tuned_digit_bits(w, h) for w, h in sizes},
-            "blend_xcd_run": handle.get_option(_lib.OPT_BLEND_XCD_RUN)}
+            "blend_xcd_run": handle.get_option(_lib.OPT_BLEND_XCD_RUN),
+            "view_branches": handle.get_option(_lib.OPT_VIEW_BRANCHES)}

@@ -1144,10 +1144,10 @@ int fhtgs_autotune(fhtgs_handle h, const fhtgs_scene* scenes, int32_t n_scenes, 
         for (auto& t : h->tuned_bits)
             if (t.first == ns) t.second = best_bits;
     }
+    std::vector<int32_t> all;
+    for (int v = 0; v < n_views; ++v) all.push_back(v);
     // blend tile runs per XCD, over the whole batch
     if (!rc) {
-        std::vector<int32_t> all;
-        for (int v = 0; v < n_views; ++v) all.push_back(v);
         int best_run = h->xcd_run;
         float best = 1e30f;
         for (int run : {0, 16, 64}) {
@@ -1157,6 +1157,21 @@ int fhtgs_autotune(fhtgs_handle h, const fhtgs_scene* scenes, int32_t n_scenes, 
             if (ms < best) { best = ms; best_run = run; }
         }
         h->xcd_run = best_run;
+    }
+    // concurrent view branches, over the whole batch: more branches overlap
+    // more of the views' latency-bound instance and sort kernels, and compete
+    // for the same CUs in their blends
+    if (!rc && n_views > 1) {
+        const int keep = h->branches;
+        int best_nb = keep;
+        float best = 1e30f;
+        for (int nb = 2; nb <= kMaxBranches; ++nb) {
+            h->branches = nb;
+            float ms = 0.0f;
+            if ((rc = timed(all, ms))) break;
+            if (ms < best) { best = ms; best_nb = nb; }
+        }
+        h->branches = rc ? keep : best_nb;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

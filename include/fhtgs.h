@@ -176,9 +176,14 @@ int fhtgs_render_batch(fhtgs_handle h, const fhtgs_scene* scenes, int32_t n_scen
 /* Autotuner for a batch (no reference counterpart): renders it once exactly,
  * then times device-count frames (best of `frames`) to pick, per view size,
  * the sort's digit width (8 or 11 bits) and, for the batch, the blend's XCD
- * run (0, 16 or 64); the choices stay in the handle. Only parameters that
- * leave every image unchanged are tuned: tile size and K are part of the
- * reference's rendering definition. Leaves the batch's images rendered. */
+ * run (0, 16 or 64) and the number of concurrent view branches (2 to 4,
+ * FHTGS_OPT_VIEW_BRANCHES); the choices stay in the handle. Only parameters
+ * that leave every image unchanged are tuned: K changes which fragments
+ * enter the core, and the 16-px tiles define the foveation classes and the
+ * sample grid. The blend's work tile (which pixels one workgroup renders
+ * from one staged list) would leave images unchanged but is fixed at 8x8:
+ * wider work tiles that share one staging measured as no gain (see the
+ * audit's --waves report). Leaves the batch's images rendered. */
 int fhtgs_autotune(fhtgs_handle h, const fhtgs_scene* scenes, int32_t n_scenes,
                    const fhtgs_view* views, const int32_t* view_scene, const int32_t* view_slot,
                    int32_t n_views, int32_t frames, void* stream);
