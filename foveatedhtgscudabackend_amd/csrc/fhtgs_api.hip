@@ -443,13 +443,17 @@ int enqueue_phase1(fhtgs_context* h, const std::vector<PreGroup>& groups, const 
     int rc = 0;
     const int nb = timed ? 1 : std::min(h->branches, n_views + 1);
     if (timed && (rc = record(h, 0, s))) return rc;
+    std::vector<const ViewParams*> cvp((size_t)n_views);
+    std::vector<const ViewBuffers*> cvb((size_t)n_views);
     for (int v = 0; v < n_views; ++v) {
         ViewSlot& sl = *h->cur[v];
         if (sl.auto_tables)
             HCHK(launch_build_tables(nullptr, sl.own_mask, sl.vp.gw, sl.vp.gh, sl.vp.fovea_r,
                                      const_cast<uint32_t*>(sl.vp.vis_mask), sl.vb.tab_vis_sat, sl.vb.tab_fovea_sat, s));
-        HCHK(launch_clear_frame(sl.vp, sl.vb, s));
+        cvp[(size_t)v] = &sl.vp;
+        cvb[(size_t)v] = &sl.vb;
     }
+    HCHK(launch_clear_frames(cvp.data(), cvb.data(), n_views, s));
     if (timed && (rc = record(h, 1, s))) return rc;
     auto tiles = [&]() -> int {
         for (int v = 0; v < n_views; ++v) {

@@ -41,6 +41,20 @@ __device__ __forceinline__ v3 normalize3(v3 v) {
 // quotient for every float with 2^-125 <= |x| <= 2^125 on gfx950 (checked
 // exhaustively by rcp_check, built next to the library); other x take the
 // IEEE division.
+// fh_expf_m64 with its last step in gfx950 form, bit-identical on [-64, 0]
+// and NaN (csrc/rcp_check.hip compares the two on every such input): the
+// scaling by 2^k is v_ldexp_f32, exact for these normal results as the
+// multiply by the power of two is; for NaN x, p is NaN and so is the result,
+// whatever v_cvt_i32_f32 gives for k (the portable form's select is not
+// needed).
+__device__ __forceinline__ float fh_expf_m64_gfx(float x) {
+    float k;
+    const float p = fh_expf_core(x, &k);
+    int32_t ki;
+    __asm__("v_cvt_i32_f32 %0, %1" : "=v"(ki) : "v"(k));
+    return __builtin_amdgcn_ldexpf(p, ki);
+}
+
 __device__ __forceinline__ float rcp_rn(float x) {
     const float r0 = __builtin_amdgcn_rcpf(x);
     float r = fmaf(fmaf(-x, r0, 1.0f), r0, r0);

@@ -145,6 +145,9 @@ struct InstanceBuffers {
 // partition) from the frame-info words; the host only sizes the grids, either
 // exactly (after reading the counts back) or to the buffers' capacity.
 hipError_t launch_clear_frame(const ViewParams& vp, const ViewBuffers& vb, hipStream_t s);
+// The same for n_views views, as few launches as possible.
+hipError_t launch_clear_frames(const ViewParams* const* vps, const ViewBuffers* const* vbs, int n_views,
+                               hipStream_t s);
 hipError_t launch_tiles(const ViewParams& vp, const ViewParams* dvp, const ViewBuffers& vb, hipStream_t s);
 hipError_t launch_preprocess(const ViewParams* const* vps, const ViewParams* const* dvps,
                              const ViewBuffers* vbs, int n_views, const SceneArgs* dscene,

@@ -95,13 +95,9 @@ FH_HD float fh_expf(float x) {
     return fh_ldexp_small(p, (int32_t)k);
 }
 
-// fh_expf for x in [-64, 0] or NaN, without the range checks: the same
-// operations, and the final 2^k scaling (exact in both forms) done in one
-// multiply, so the result is bit-identical to fh_expf there. The blend's
-// exponent -rho^2/2 is >= -5.55 (rho^2 <= 11.0825) or NaN.
-// fh_expf on [-64, 0] and NaN without the range branch (the blend's
-// exponent is -rho^2/2 with rho^2 <= 11.08, so it never leaves [-6, 0]).
-FH_HD float fh_expf_m64(float x) {
+// The reduction and polynomial of fh_expf: e^x = p * 2^k, k = rint(x / ln2)
+// (the same operations as fh_expf, without its range checks).
+FH_HD float fh_expf_core(float x, float* k_out) {
     const float k = rintf(x * 0x1.715476p+0f);
     float r = fh_fma(-k, 0x1.62e400p-1f, x);
     r = fh_fma(-k, 0x1.7f7d1cp-20f, r);
@@ -113,6 +109,18 @@ FH_HD float fh_expf_m64(float x) {
     p = fh_fma(p, r, 0.5f);
     p = fh_fma(p, r, 1.0f);
     p = fh_fma(p, r, 1.0f);
+    *k_out = k;
+    return p;
+}
+
+// fh_expf for x in [-64, 0] or NaN, without the range checks: the same
+// operations, and the final 2^k scaling (exact in both forms: p is within
+// [0.7, 1.42] and k >= -93, so p * 2^k is a normal float) done in one
+// multiply, so the result is bit-identical to fh_expf there. The blend's
+// exponent -rho^2/2 is >= -5.55 (rho^2 <= 11.0825) or NaN.
+FH_HD float fh_expf_m64(float x) {
+    float k;
+    const float p = fh_expf_core(x, &k);
     const int32_t ki = (k == k) ? (int32_t)k : 0;
     return p * fh_pow2i(ki);
 }

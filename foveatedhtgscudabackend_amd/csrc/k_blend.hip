@@ -315,7 +315,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 5 :
             if (num > kMaxCutoffSq * den) continue;
             // num <= 11.08 den here, so the exponent lies in [-6, 0] (or is NaN)
             const float rcp = rcp_rn(den);
-            const float G = fh_expf_m64((-0.5f * num) * rcp);
+            const float G = fh_expf_m64_gfx((-0.5f * num) * rcp);
             const float4 Cj = sC[e];
             const float alpha = fminf(Cj.w * G, kMaxFragmentAlpha);
             if (alpha < kMinAlpha) continue;

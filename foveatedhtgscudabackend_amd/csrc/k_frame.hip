@@ -6,23 +6,55 @@
 namespace fhtgs {
 
 // Zeroes the instance-range table (inference.cu:124), the frame-info words
-// and the blend work order's bucket counters.
-__global__ __launch_bounds__(256) void clear_frame_kernel(uint2* ranges, uint32_t n_small,
-                                                          uint32_t* frame_info, uint32_t* order_counts) {
+// and the blend work order's bucket counters of up to kClearViews views, one
+// view per grid row: a frame's views are cleared by one launch, which is on
+// the way to its first preprocess.
+constexpr int kClearViews = 8;
+struct ClearArgs {
+    uint2* ranges[kClearViews];
+    uint32_t* frame_info[kClearViews];
+    uint32_t* order_counts[kClearViews];
+    uint32_t n_small[kClearViews];
+};
+
+__global__ __launch_bounds__(256) void clear_frame_kernel(ClearArgs a) {
+    const uint32_t v = blockIdx.y;
+    const uint32_t n_small = a.n_small[v];
+    uint2* ranges = a.ranges[v];
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_small; i += stride)
         ranges[i] = make_uint2(0u, 0u);
-    if (blockIdx.x == 0 && threadIdx.x < FI_WORDS) frame_info[threadIdx.x] = 0u;
-    if (blockIdx.x == 0 && threadIdx.x < 128u) order_counts[threadIdx.x] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x < FI_WORDS) a.frame_info[v][threadIdx.x] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x < 128u) a.order_counts[v][threadIdx.x] = 0u;
+}
+
+hipError_t launch_clear_frames(const ViewParams* const* vps, const ViewBuffers* const* vbs, int n_views,
+                               hipStream_t s) {
+    for (int v0 = 0; v0 < n_views; v0 += kClearViews) {
+        const int nv = std::min(kClearViews, n_views - v0);
+        ClearArgs a;
+        uint32_t most = 1;
+        for (int k = 0; k < kClearViews; ++k) {
+            const int v = v0 + (k < nv ? k : 0);
+            a.ranges[k] = vbs[v]->ranges;
+            a.frame_info[k] = vbs[v]->frame_info;
+            a.order_counts[k] = vbs[v]->order_counts;
+            a.n_small[k] = (uint32_t)vps[v]->n_small;
+            most = std::max(most, a.n_small[k]);
+        }
+        uint32_t blocks = (most + 255u) / 256u;
+        if (blocks > 1024u) blocks = 1024u;
+        clear_frame_kernel<<<dim3(blocks, (unsigned)nv), 256, 0, s>>>(a);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_clear_frame(const ViewParams& vp, const ViewBuffers& vb, hipStream_t s) {
-    const uint32_t n = (uint32_t)vp.n_small;
-    uint32_t blocks = (n + 255u) / 256u;
-    if (blocks < 1u) blocks = 1u;
-    if (blocks > 1024u) blocks = 1024u;
-    clear_frame_kernel<<<blocks, 256, 0, s>>>(vb.ranges, n, vb.frame_info, vb.order_counts);
-    return hipGetLastError();
+    const ViewParams* vps[1] = {&vp};
+    const ViewBuffers* vbs[1] = {&vb};
+    return launch_clear_frames(vps, vbs, 1, s);
 }
 
 // Block-wide exclusive scan of four counters (1024 threads = 16 waves):

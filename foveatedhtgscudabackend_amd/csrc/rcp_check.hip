@@ -1,7 +1,9 @@
-// rcp_check: exhaustive check that rcp_rn (fhtgs_device.h), the blend's
-// reciprocal, returns the IEEE-754 quotient 1/x bit for bit for every one of
-// the 2^32 float bit patterns (NaN results compared as NaN). Prints the number
-// of mismatches and the first few; exit status 0 iff there are none.
+// rcp_check: exhaustive checks of the blend's two gfx950 fast paths
+// (fhtgs_device.h): rcp_rn returns the IEEE-754 quotient 1/x bit for bit for
+// every one of the 2^32 float bit patterns, and fh_expf_m64_gfx equals the
+// portable fh_expf_m64 bit for bit on every float of [-64, 0] (NaN results
+// compared as NaN). Prints the number of mismatches and the first few; exit
+// status 0 iff there are none.
 #include "fhtgs_device.h"
 
 #include <cstdio>
@@ -17,7 +19,14 @@ __global__ __launch_bounds__(256) void rcp_check_kernel(unsigned long long* coun
         const float want = 1.0f / x;
         const float got = fhtgs::rcp_rn(x);
         const bool both_nan = want != want && got != got;
-        if (!both_nan && __float_as_uint(want) != __float_as_uint(got)) {
+        bool bad = !both_nan && __float_as_uint(want) != __float_as_uint(got);
+        // the exp fast path on [-64, 0] (including -0) and every NaN
+        if ((x <= 0.0f && x >= -64.0f) || x != x) {
+            const float e0 = fh_expf_m64(x), e1 = fhtgs::fh_expf_m64_gfx(x);
+            const bool e_nan = e0 != e0 && e1 != e1;
+            bad |= !e_nan && __float_as_uint(e0) != __float_as_uint(e1);
+        }
+        if (bad) {
             const unsigned long long k = atomicAdd(count, 1ull);
             if (k < 8ull) first[k] = (uint32_t)i;
         }

@@ -1,6 +1,9 @@
 """The blend computes 1/den as v_rcp_f32 plus one FMA Newton step (rcp_rn in
-csrc/fhtgs_device.h) and relies on it being the IEEE quotient; rcp_check
-compares the two over all 2^32 float inputs on the GPU."""
+csrc/fhtgs_device.h) and relies on it being the IEEE quotient, and scales its
+exponential with v_ldexp_f32 (fh_expf_m64_gfx) where the oracle multiplies
+by a power of two; rcp_check compares rcp_rn with IEEE 1/x over all 2^32
+float inputs, and the two exponentials on every float of [-64, 0] and NaN,
+on the GPU."""
 import os
 import subprocess
 
