@@ -285,7 +285,9 @@ def main() -> int:
     sr = S.settings_for(right, gaze, device=dev, K=args.K, foveation_radius_px=fr, blend_width_px=bw)
     rast = FoveatedHTGSRasterizer(fr, bw, qr)
     handle = _lib.handle_for(torch.cuda.current_device())
-    handle.set_option(_lib.OPT_STAGE_TIMING, 1)
+    # the timed and latency frames run without the stage-timing events
+    # (a profiling aid, off by default); stage_profile turns them on
+    handle.set_option(_lib.OPT_STAGE_TIMING, 0)
     if args.xcd_run is not None:
         handle.set_option(_lib.OPT_BLEND_XCD_RUN, args.xcd_run)
     if args.view_branches is not None:
@@ -330,6 +332,7 @@ def main() -> int:
         device-count mode launches exactly what the graph replays, one event
         pair around each stage."""
         handle.set_option(_lib.OPT_FRAME_MODE, _lib.FRAME_DEVICE)
+        handle.set_option(_lib.OPT_STAGE_TIMING, 1)
         fn()
         torch.cuda.synchronize()
         stages = []
@@ -337,6 +340,7 @@ def main() -> int:
             fn()
             torch.cuda.synchronize()
             stages.append(handle.stage_times())
+        handle.set_option(_lib.OPT_STAGE_TIMING, 0)
         handle.set_option(_lib.OPT_FRAME_MODE, _lib.FRAME_GRAPH)
         return stages
 

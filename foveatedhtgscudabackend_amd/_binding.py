@@ -14,15 +14,17 @@ from . import _lib
 
 
 def _dev_ptr(t: torch.Tensor, name: str, dtypes, device) -> tuple[int, torch.Tensor]:
+    # called for every tensor of every frame: the cheapest checks first
     if not isinstance(t, torch.Tensor):
         raise TypeError(f"{name} must be a torch.Tensor")
     if t.dtype not in dtypes:
         raise TypeError(f"{name} must have dtype in {dtypes}, got {t.dtype}")
-    if t.device.type != "cuda":
+    if not t.is_cuda:
         raise ValueError(f"{name} must be a GPU tensor")
-    if device is not None and t.device != device:
+    if device is not None and t.get_device() != device.index:
         raise ValueError(f"{name} is on {t.device}, expected {device}")
-    t = t.contiguous()
+    if not t.is_contiguous():
+        t = t.contiguous()
     p = t.data_ptr()
     if p % 16:
         t = t.clone()  # vector loads need 16-byte aligned rows
