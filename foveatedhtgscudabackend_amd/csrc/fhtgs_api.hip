@@ -133,6 +133,7 @@ struct fhtgs_context {
     int branches = 2;
     hipStream_t side[kMaxBranches - 1] = {};
     hipEvent_t ev_fork = nullptr;
+    hipEvent_t ev_params = nullptr;  // the parameter upload, on side stream 0
     hipEvent_t ev_join[kMaxBranches - 1] = {};
     hipEvent_t ev[FHTGS_NUM_STAGES + 1] = {};
     float stage_ms[FHTGS_NUM_STAGES] = {};
@@ -430,16 +431,17 @@ int join_branches(fhtgs_context* h, int nb, hipStream_t s) {
     return FHTGS_OK;
 }
 
-// Phase 1 of a frame: clears, tile classification, preprocess, scan.
-// dscenes[k]: device address of scene k's SceneArgs in the parameter block.
-// Untimed, with branches, the tile classifications (one workgroup each) run
-// on the side streams beside the preprocess, and the scans side by side
-// after it; they are joined at the end. sort_clear: the instance buffers are
-// already sized (device counts, graph), so their sort counters are cleared
-// here, after each view's tile classification, instead of in phase 2.
+// Phase 1 of a frame: parameter upload, clears, tile classification,
+// preprocess, scan. pbytes: the parameter block's size. Untimed, with
+// branches, the upload and the tile classifications (one workgroup each) run
+// on the side streams beside the preprocess (which takes its parameters as
+// kernel arguments), and the scans side by side after it; they are joined at
+// the end, before anything that reads the uploaded block. sort_clear: the
+// instance buffers are already sized (device counts, graph), so their sort
+// counters are cleared here, after each view's tile classification, instead
+// of in phase 2.
 int enqueue_phase1(fhtgs_context* h, const std::vector<PreGroup>& groups, const fhtgs_scene* scenes,
-                   const std::vector<const SceneArgs*>& dscenes, int n_views, bool timed, bool sort_clear,
-                   hipStream_t s) {
+                   size_t pbytes, int n_views, bool timed, bool sort_clear, hipStream_t s) {
     int rc = 0;
     const int nb = timed ? 1 : std::min(h->branches, n_views + 1);
     if (timed && (rc = record(h, 0, s))) return rc;
@@ -467,21 +469,27 @@ int enqueue_phase1(fhtgs_context* h, const std::vector<PreGroup>& groups, const 
     auto preprocess = [&]() -> int {
         for (const PreGroup& pg : groups) {
             const ViewParams* hv[2] = {&(*h->cur[pg.v[0]]).vp, &(*h->cur[pg.v[pg.nv - 1]]).vp};
-            const ViewParams* dv[2] = {(*h->cur[pg.v[0]]).dvp, (*h->cur[pg.v[pg.nv - 1]]).dvp};
             const ViewBuffers vbs[2] = {(*h->cur[pg.v[0]]).vb, (*h->cur[pg.v[pg.nv - 1]]).vb};
-            HCHK(launch_preprocess(hv, dv, vbs, pg.nv, dscenes[pg.scene], (uint32_t)scenes[pg.scene].n_primitives,
-                                   s));
+            const fhtgs_scene& sc = scenes[pg.scene];
+            const SceneArgs sa{sc.positions, sc.scales, sc.rotations, sc.opacities, sc.sh_0, sc.sh_rest};
+            HCHK(launch_preprocess(hv, vbs, pg.nv, sa, (uint32_t)sc.n_primitives, s));
         }
         return FHTGS_OK;
     };
     if (nb > 1) {
-        // the preprocess is enqueued before the tile kernels, which only
-        // wait for the clears
+        // the preprocess is enqueued first; the upload and then the tile
+        // kernels follow on the side streams, after the clears
         HCHK(hipEventRecord(h->ev_fork, s));
         if ((rc = preprocess())) return rc;
-        for (int b = 1; b < nb; ++b) HCHK(hipStreamWaitEvent(h->side[b - 1], h->ev_fork, 0));
+        HCHK(hipStreamWaitEvent(h->side[0], h->ev_fork, 0));
+        HCHK(hipMemcpyAsync(h->params.ptr, h->params_host, pbytes, hipMemcpyHostToDevice, h->side[0]));
+        if (nb > 2) {
+            HCHK(hipEventRecord(h->ev_params, h->side[0]));
+            for (int b = 2; b < nb; ++b) HCHK(hipStreamWaitEvent(h->side[b - 1], h->ev_params, 0));
+        }
         if ((rc = tiles())) return rc;
     } else {
+        HCHK(hipMemcpyAsync(h->params.ptr, h->params_host, pbytes, hipMemcpyHostToDevice, s));
         if ((rc = tiles())) return rc;
         if (timed && (rc = record(h, 2, s))) return rc;
         if ((rc = preprocess())) return rc;
@@ -636,10 +644,11 @@ int render_impl(fhtgs_context* h, const fhtgs_scene* scenes, int n_scenes, const
     h->cur.resize((size_t)n_views);
     for (int v = 0; v < n_views; ++v) h->cur[(size_t)v] = &h->slots[(size_t)(view_slot ? view_slot[v] : v)];
 
-    // ---- parameters: host copies, then one upload
+    // ---- parameters: host copies, then one upload (the preprocess takes its
+    // views' parameters and the scene as kernel arguments; the other kernels
+    // read them from this block, so a captured graph replays with new ones)
     const size_t vp_stride = align_up(sizeof(ViewParams));
-    const size_t sc_stride = align_up(sizeof(SceneArgs));
-    const size_t pbytes = vp_stride * (size_t)n_views + sc_stride * (size_t)n_scenes;
+    const size_t pbytes = vp_stride * (size_t)n_views;
     int rc = ensure(h, h->params, pbytes);
     if (rc) return rc;
     if (h->params_host_cap < pbytes) {
@@ -652,16 +661,6 @@ int render_impl(fhtgs_context* h, const fhtgs_scene* scenes, int n_scenes, const
     }
     char* ph = (char*)h->params_host;
     char* pd = (char*)h->params.ptr;
-    // scene k at byte offset vp_stride * n_views + sc_stride * k, in the
-    // staging copy and on the device alike
-    std::vector<const SceneArgs*> dscenes((size_t)n_scenes);
-    for (int k = 0; k < n_scenes; ++k) {
-        const fhtgs_scene& sc = scenes[k];
-        const SceneArgs a{sc.positions, sc.scales, sc.rotations, sc.opacities, sc.sh_0, sc.sh_rest};
-        const size_t off = vp_stride * (size_t)n_views + sc_stride * (size_t)k;
-        std::memcpy(ph + off, &a, sizeof(a));
-        dscenes[(size_t)k] = (const SceneArgs*)(pd + off);
-    }
     for (int v = 0; v < n_views; ++v) {
         ViewSlot& slot = (*h->cur[v]);
         const fhtgs_scene& sc = scenes[view_scene[v]];
@@ -703,10 +702,9 @@ int render_impl(fhtgs_context* h, const fhtgs_scene* scenes, int n_scenes, const
             if ((rc = size_instances(h, (*h->cur[v]), (*h->cur[v]).cap_hint))) return rc;
     }
     const bool timed = h->timing != 0 && mode != kModeGraph;
-    HCHK(hipMemcpyAsync(h->params.ptr, h->params_host, pbytes, hipMemcpyHostToDevice, s));
 
     if (mode == kModeExact) {
-        if ((rc = enqueue_phase1(h, groups, scenes, dscenes, n_views, timed, false, s))) return rc;
+        if ((rc = enqueue_phase1(h, groups, scenes, pbytes, n_views, timed, false, s))) return rc;
         for (int v = 0; v < n_views; ++v)
             HCHK(hipMemcpyAsync((*h->cur[v]).host_fi, (*h->cur[v]).vb.frame_info, FI_WORDS * sizeof(uint32_t),
                                 hipMemcpyDeviceToHost, s));
@@ -729,14 +727,14 @@ int render_impl(fhtgs_context* h, const fhtgs_scene* scenes, int n_scenes, const
         }
         if ((rc = enqueue_phase2(h, views, n_views, true, timed, s))) return rc;
     } else if (mode == kModeDevice) {
-        if ((rc = enqueue_phase1(h, groups, scenes, dscenes, n_views, timed, true, s))) return rc;
+        if ((rc = enqueue_phase1(h, groups, scenes, pbytes, n_views, timed, true, s))) return rc;
         if (timed && (rc = record(h, 4, s))) return rc;
         if ((rc = enqueue_phase2(h, views, n_views, false, timed, s))) return rc;
     } else {
         if (h->timing) HCHK(hipEventRecord(h->ev[0], s));
         // phase 1 straight onto the stream (its first kernels start at once);
         // the captured graph holds phase 2 and is launched while phase 1 runs
-        if ((rc = enqueue_phase1(h, groups, scenes, dscenes, n_views, false, true, s))) return rc;
+        if ((rc = enqueue_phase1(h, groups, scenes, pbytes, n_views, false, true, s))) return rc;
         std::vector<int64_t> sig = launch_signature(h, groups, scenes, views, n_views);
         GraphEntry* ge = nullptr;
         for (GraphEntry& e : h->graphs)
@@ -858,7 +856,8 @@ int fhtgs_create(int device, fhtgs_handle* out) {
         }
     }
     bool ok = hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking) == hipSuccess &&
-              hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) == hipSuccess;
+              hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&h->ev_params, hipEventDisableTiming) == hipSuccess;
     for (int b = 0; ok && b < kMaxBranches - 1; ++b)
         ok = hipStreamCreateWithFlags(&h->side[b], hipStreamNonBlocking) == hipSuccess &&
              hipEventCreateWithFlags(&h->ev_join[b], hipEventDisableTiming) == hipSuccess;
@@ -898,6 +897,7 @@ int fhtgs_destroy(fhtgs_handle h) {
         if (h->ev_join[b]) (void)hipEventDestroy(h->ev_join[b]);
     }
     if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
+    if (h->ev_params) (void)hipEventDestroy(h->ev_params);
     if (h->done) (void)hipEventDestroy(h->done);
     for (int i = 0; i <= FHTGS_NUM_STAGES; ++i)
         if (h->ev[i]) (void)hipEventDestroy(h->ev[i]);

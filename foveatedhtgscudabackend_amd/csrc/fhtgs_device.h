@@ -93,17 +93,6 @@ __device__ __forceinline__ ViewParams load_view(const ViewParams* src) {
     return v;
 }
 
-__device__ __forceinline__ SceneArgs load_scene(const SceneArgs* src) {
-    SceneArgs s;
-    s.pos = load_gptr(&src->pos);
-    s.scl = load_gptr(&src->scl);
-    s.rot = load_gptr(&src->rot);
-    s.opa = load_gptr(&src->opa);
-    s.sh0 = load_gptr(&src->sh0);
-    s.shr = load_gptr(&src->shr);
-    return s;
-}
-
 __device__ __forceinline__ v4 load_row(const float* m, int row) {
     return v4{m[4 * row + 0], m[4 * row + 1], m[4 * row + 2], m[4 * row + 3]};
 }

@@ -85,8 +85,8 @@ struct ViewParams {
     float* image_blend;   // periphery / blended blend target (temp image when blurring)
 };
 
-// Scene tensors of one preprocess launch (device resident, like ViewParams,
-// so that a captured frame can be replayed with other inputs).
+// Scene tensors of one preprocess launch, passed by value in its kernel
+// arguments (the preprocess is not part of a captured graph).
 struct SceneArgs {
     const float* pos;
     const float* scl;
@@ -149,9 +149,9 @@ hipError_t launch_clear_frame(const ViewParams& vp, const ViewBuffers& vb, hipSt
 hipError_t launch_clear_frames(const ViewParams* const* vps, const ViewBuffers* const* vbs, int n_views,
                                hipStream_t s);
 hipError_t launch_tiles(const ViewParams& vp, const ViewParams* dvp, const ViewBuffers& vb, hipStream_t s);
-hipError_t launch_preprocess(const ViewParams* const* vps, const ViewParams* const* dvps,
-                             const ViewBuffers* vbs, int n_views, const SceneArgs* dscene,
-                             uint32_t n, hipStream_t s);
+// The views' parameters and the scene go by value into the kernel arguments.
+hipError_t launch_preprocess(const ViewParams* const* vps, const ViewBuffers* vbs, int n_views,
+                             const SceneArgs& scene, uint32_t n, hipStream_t s);
 hipError_t launch_scan_blocks(const ViewBuffers& vb, uint32_t n_blocks, hipStream_t s);
 hipError_t launch_sort_clear(const InstanceBuffers& ib, const SortPlan& plan, int64_t tiles,
                              hipStream_t s);

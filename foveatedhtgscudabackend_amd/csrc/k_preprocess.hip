@@ -19,11 +19,14 @@ namespace fhtgs {
 constexpr int kMaxViewsPerPass = 2;
 constexpr int kShChunk = 24;  // visible Gaussians per SH gather round (8, 16, 32 measured slower)
 
-// Kernel argument: device pointers to the views' parameters and the scene.
+// Kernel argument: the views' parameters and the scene's tensors by value
+// (pointers read from kernel arguments are global ones to the compiler). The
+// preprocess therefore does not wait for the frame's parameter upload, which
+// only the later kernels read.
 struct PreprocessArgs {
-    const ViewParams* vp[kMaxViewsPerPass];
+    ViewParams vp[kMaxViewsPerPass];
     ViewBuffers vb[kMaxViewsPerPass];
-    const SceneArgs* scene;
+    SceneArgs scene;
     uint32_t n;
 };
 
@@ -45,10 +48,10 @@ __device__ __forceinline__ PreprocessState resolve(const PreprocessArgs& g) {
     PreprocessState a;
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
-        a.vp[v] = load_view(g.vp[v]);
+        a.vp[v] = g.vp[v];
         a.vb[v] = g.vb[v];
     }
-    const SceneArgs sc = load_scene(g.scene);
+    const SceneArgs& sc = g.scene;
     a.pos = sc.pos; a.scl = sc.scl; a.rot = sc.rot; a.opa = sc.opa; a.sh0 = sc.sh0; a.shr = sc.shr;
     a.n = g.n;
     return a;
@@ -590,18 +593,17 @@ __global__ __launch_bounds__(kPrimBlock) void preprocess_kernel(PreprocessArgs g
     }
 }
 
-hipError_t launch_preprocess(const ViewParams* const* vps, const ViewParams* const* dvps,
-                             const ViewBuffers* vbs, int n_views, const SceneArgs* dscene,
-                             uint32_t n, hipStream_t s) {
+hipError_t launch_preprocess(const ViewParams* const* vps, const ViewBuffers* vbs, int n_views,
+                             const SceneArgs& scene, uint32_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
     if (n_views < 1 || n_views > kMaxViewsPerPass) return hipErrorInvalidValue;
     PreprocessArgs a;
     for (int v = 0; v < kMaxViewsPerPass; ++v) {
         const int k = v < n_views ? v : 0;
-        a.vp[v] = dvps[k];
+        a.vp[v] = *vps[k];
         a.vb[v] = vbs[k];
     }
-    a.scene = dscene;
+    a.scene = scene;
     a.n = n;
     const uint32_t blocks = (n + kPrimBlock - 1) / kPrimBlock;
     // inference.cuh:86-112: the anti-aliased variant replaces transform and cull
