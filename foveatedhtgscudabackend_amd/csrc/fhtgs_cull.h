@@ -162,3 +162,65 @@ FH_HD bool fh_pixel_in_conic(const float k[6], float u, float v) {
     const float du = u - k[0], dv = v - k[1];
     return fmaf(k[2], du * du, fmaf(k[4], dv * dv, k[3] * (du * dv))) <= k[5];
 }
+
+// ---- sample masks: which samples of a tile an ellipse record can reach
+//
+// The samples of a tile are (bx + c step, by + r step), c < cols, r < rows
+// (8 x 8 at stride 1 or 2, 4 x 4 at stride 4); bit r * cols + c of the mask
+// stands for sample (c, r). In row r the record's ellipse spans
+//     x = x_c(y) +- w(y),  x_c(y) = cx + (m12 / m22)(y - cy),
+//     w(y)^2 = det / m22^2 * (m22 - (y - cy)^2),
+// so the row's bits are the columns of [x_c - w, x_c + w] widened by
+// kMaskMarginPx on each side. The blend's staging lane computes one mask per
+// staged Gaussian (the square root with the hardware's v_sqrt_f32); the
+// pixels' first pass then reads one bit per staged Gaussian instead of
+// testing a conic. Records that are flagged, or too eccentric for det to be
+// accurate (det <= 1e-4 m11 m22, the same rule as the pixel conic), get
+// every sample. tests/cull_audit.cpp checks, with the square root lowered
+// by a relative 2^-20, that the masks keep every sample the reference's
+// cutoff and alpha tests keep.
+constexpr float kMaskMarginPx = 0.015625f;
+
+struct FhMaskPrep {
+    float u0;     // (cx - bx) / step: the centre in sample columns
+    float dy0;    // by - cy: the first row's offset from the centre
+    float s;      // m12 / m22 / step: column shift per pixel of dy
+    float a;      // det / m22^2 / step^2: w(y)^2 in columns^2 is a (m22 - dy^2)
+    float m22;
+    bool all;     // no usable ellipse: every sample
+};
+
+// inv22 and inv_step: 1 / m22 and 1 / step, correctly rounded (the kernels
+// use a reciprocal that equals IEEE division on every float).
+FH_HD FhMaskPrep fh_mask_prep(const float r[5], float inv22, float bx, float by, float inv_step) {
+    FhMaskPrep p;
+    const float m11 = r[2], m22 = r[3], m12 = r[4];
+    const float det = fmaf(m11, m22, -(m12 * m12));
+    p.all = !(m11 > 0.0f) || !(det > 1e-4f * (m11 * m22));
+    p.u0 = (r[0] - bx) * inv_step;
+    p.dy0 = by - r[1];
+    p.s = (m12 * inv22) * inv_step;
+    p.a = ((det * inv22) * inv22) * (inv_step * inv_step);
+    p.m22 = m22;
+    return p;
+}
+
+// w(y)^2 in columns^2 for row offset row_dy (negative: the row misses).
+FH_HD float fh_mask_row_w2(const FhMaskPrep& p, float row_dy) {
+    return p.a * fmaf(-row_dy, row_dy, p.m22);
+}
+
+// The column bits (at most 8 columns) of the row at y offset row_dy from the
+// centre, given w, the square root of fh_mask_row_w2 in columns; the caller
+// passes -1 when that is negative (the row misses). A NaN w or NaN bound
+// gives the whole row: fmax / fmin return their other operand.
+FH_HD uint32_t fh_mask_row_bits(const FhMaskPrep& p, float row_dy, float w, int cols) {
+    const float uc = fmaf(p.s, row_dy, p.u0);
+    const float lo = fh_fmax(ceilf((uc - w) - kMaskMarginPx), 0.0f);
+    const float hi = fh_fmin(floorf((uc + w) + kMaskMarginPx), (float)(cols - 1));
+    // both converted inside [0, cols - 1]; the bits count only if lo <= hi
+    const uint32_t il = (uint32_t)fh_fmin(lo, (float)(cols - 1));
+    const uint32_t ih = (uint32_t)fh_fmax(hi, 0.0f);
+    const uint32_t bits = (2u << ih) - (1u << il);
+    return (!(w < 0.0f) && !(lo > hi)) ? bits : 0u;
+}

@@ -11,7 +11,9 @@
 // samples), four tiles per wave: 16-lane groups, each walking its own tile's
 // range in batches of 16 through its own quarter of the LDS batch.
 // Per LDS batch each lane first marks the staged Gaussians that can reach its
-// pixel, then evaluates only those, in order (two passes, same arithmetic).
+// pixel -- one bit of each staged Gaussian's sample mask, which its staging
+// lane computed from the support ellipse row by row -- then evaluates only
+// those, in order (two passes, same arithmetic).
 // Workgroups take their tiles from a work order with the longest instance
 // lists first (blend_order_count_kernel, blend_order_place_kernel).
 #include "fhtgs_device.h"
@@ -147,10 +149,10 @@ template <int K>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 5 : 1))) void blend_kernel(const ViewParams* vpp, BlendArgs a) {
     const ViewParams vp = load_view(vpp);
     __shared__ float4 sV1[64], sV2[64], sV4[64], sC[64];
-    // first-pass data per staged Gaussian: pixel conic (centre, form) and its
-    // bound, then the exact test's bound on rho^2 (fh_ellipse_pixel_conic,
-    // fh_pixel_cutoff)
-    __shared__ float4 sQ1[64], sQ2[64];
+    // first-pass data per staged Gaussian: its sample mask as two words
+    // (bit row * cols + col, fh_mask_row_bits), or every sample for a record
+    // without a usable ellipse
+    __shared__ uint32_t sM[128];
     const uint32_t* fi = a.frame_info;
     const uint32_t g0 = xcd_remap(blockIdx.x, gridDim.x, a.xcd_run);
     const uint32_t nq = fi[7] - fi[6];
@@ -187,9 +189,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 5 :
     const float pxf = (float)px, pyf = (float)py;
     const float tx0 = (float)bx, tx1 = (float)(bx + (cols - 1u) * step);
     const float ty0 = (float)by, ty1 = (float)(by + (cols - 1u) * step);
-    // the tile centre and this lane's pixel relative to it (first pass)
-    const float ox = 0.5f * (tx0 + tx1), oy = 0.5f * (ty0 + ty1);
-    const float pu = pxf - ox, pv = pyf - oy;
+    const float stepf = (float)step, inv_step = quarter ? 0.25f : (step == 2u ? 0.5f : 1.0f);
+    // this lane's bit in its group's sample masks (first pass)
+    const uint32_t m_word = li >> 5, m_shift = li & 31u;
     const uint64_t lt_mask = lanemask_lt() & gmask;
 
     float cd[K];
@@ -209,6 +211,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 5 :
         bool keep = false;
         float4 V1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), V2 = V1, V4 = V1, C = V1;
         float er[5] = {0.0f, 0.0f, -1.0f, 0.0f, 0.0f};
+        float inv22 = 0.0f;
         if (idx < range.y) {
             const uint32_t p = a.vals[idx];
             const float4 E = a.ell_a[p];
@@ -222,7 +225,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 5 :
             V2 = rec[1];
             V4 = rec[2];
             C = rec[3];
-            keep = fh_ellipse_may_touch(er, rcp_rn(er[2]), rcp_rn(er[3]), tx0, tx1, ty0, ty1);
+            inv22 = rcp_rn(er[3]);
+            keep = fh_ellipse_may_touch(er, rcp_rn(er[2]), inv22, tx0, tx1, ty0, ty1);
         }
         const uint64_t km = __ballot(keep);
         if (km == 0ull) continue;
@@ -242,49 +246,46 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 5 :
             sV2[pos] = V2;
             sV4[pos] = V4;
             sC[pos] = C;
-            // first-pass data: the pixel conic about the tile centre, or
-            // (w = 1) no usable conic: the exact test against the cutoff bound
-            float kq[6];
-            fh_ellipse_pixel_conic(er, ox, oy, kq);
-            const bool exact = kq[5] < 0.0f;
-            float cut = 0.0f;
-            if (__ballot(exact) != 0ull) cut = exact ? fh_pixel_cutoff(C.w) : 0.0f;  // rare
-            sQ1[pos] = make_float4(kq[0], kq[1], kq[2], kq[3]);
-            sQ2[pos] = make_float4(kq[4], kq[5], cut, exact ? 1.0f : 0.0f);
+            // first-pass data: which of the group's samples the support
+            // ellipse can reach, row by row (fh_mask_row_bits; every sample
+            // for a record without a usable ellipse, which the second pass's
+            // own tests then settle)
+            const FhMaskPrep mp = fh_mask_prep(er, inv22, tx0, ty0, inv_step);
+            uint32_t m0 = ~0u, m1 = quarter ? 0u : ~0u;
+            if (quarter) m0 = 0xffffu;
+            if (!mp.all) {
+                m0 = 0u;
+                m1 = 0u;
+#pragma unroll
+                for (uint32_t r = 0; r < 8u; ++r) {
+                    if (r >= cols) continue;
+                    const float dy = fmaf((float)r, stepf, mp.dy0);
+                    const float w2 = fh_mask_row_w2(mp, dy);
+                    const float w = w2 < 0.0f ? -1.0f : __builtin_amdgcn_sqrtf(w2);
+                    const uint32_t bits = fh_mask_row_bits(mp, dy, w, (int)cols);
+                    const uint32_t sh = r * cols;
+                    if (sh < 32u) m0 |= bits << sh;
+                    else m1 |= bits << (sh - 32u);
+                }
+            }
+            sM[2u * pos] = m0;
+            sM[2u * pos + 1u] = m1;
         }
         if ((int)li >= count) {
             // a slot past the group's Gaussians (any lane there, staging or
-            // not, since its Gaussian went to a lower slot): a conic no pixel
-            // is inside, for the steps of groups with fewer Gaussians than
-            // the wave's largest group
-            sQ1[gbase + li] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            sQ2[gbase + li] = make_float4(0.0f, -1.0f, 0.0f, 0.0f);
+            // not, since its Gaussian went to a lower slot): a mask with no
+            // sample, for the steps of groups with fewer Gaussians than the
+            // wave's largest group
+            sM[2u * (gbase + li)] = 0u;
+            sM[2u * (gbase + li) + 1u] = 0u;
         }
         __syncthreads();
         if (!inside) continue;
-        // Pass 1: which staged Gaussians can reach this pixel. Per Gaussian
-        // either its pixel conic or, without a usable one, the reference's num
-        // and den against fh_pixel_cutoff; both keep every pair the reference's
-        // cutoff and alpha tests keep. The answers are shifted in, first
-        // Gaussian highest: bit 63 - jj of `reach` is staged Gaussian jj.
-        auto may_reach = [&](uint32_t e) -> uint32_t {
-            const float4 q1 = sQ1[e], q2 = sQ2[e];
-            const float k[6] = {q1.x, q1.y, q1.z, q1.w, q2.x, q2.y};
-            bool r = fh_pixel_in_conic(k, pu, pv);
-            if (__ballot(q2.w != 0.0f) != 0ull) {  // rare, wave-uniform
-                const float4 W1 = sV1[e], W2 = sV2[e], W4 = sV4[e];
-                const float Px0 = fmaf(-W4.x, pxf, W1.x), Px1 = fmaf(-W4.y, pxf, W1.y);
-                const float Px2 = fmaf(-W4.z, pxf, W1.z), Px3 = fmaf(-W4.w, pxf, W1.w);
-                const float Py0 = fmaf(-W4.x, pyf, W2.x), Py1 = fmaf(-W4.y, pyf, W2.y);
-                const float Py2 = fmaf(-W4.z, pyf, W2.z), Py3 = fmaf(-W4.w, pyf, W2.w);
-                const v3 m{fmaf(Px3, Py0, -(Px0 * Py3)), fmaf(Px3, Py1, -(Px1 * Py3)),
-                           fmaf(Px3, Py2, -(Px2 * Py3))};
-                const v3 d = cross3(v3{Px0, Px1, Px2}, v3{Py0, Py1, Py2});
-                const bool ex = !(dot3(m, m) > q2.z * dot3(d, d));
-                r = q2.w != 0.0f ? ex : r;
-            }
-            return r ? 1u : 0u;
-        };
+        // Pass 1: which staged Gaussians can reach this pixel: this lane's bit
+        // of each staged Gaussian's sample mask, a superset of the pairs the
+        // reference's cutoff and alpha tests keep. The answers are shifted in,
+        // first Gaussian highest: bit 63 - jj of `reach` is staged Gaussian jj.
+        auto may_reach = [&](uint32_t e) -> uint32_t { return (sM[2u * e + m_word] >> m_shift) & 1u; };
         const int na = nmax < 32 ? nmax : 32;
         uint32_t ra = 0u, rb = 0u;
 #pragma unroll 4

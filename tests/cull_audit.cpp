@@ -2,13 +2,16 @@
 // record fh_ellipse_record that the preprocess stores): over every (tile,
 // instance) pair of a frame, a Gaussian that the cull drops must reach no
 // sample of the tile, evaluated with the blend's own float32 operations
-// (cutoff, exp, alpha >= 1/255). Then, per pixel, the blend's first-pass
-// pre-test (fh_ellipse_pixel_conic / fh_pixel_in_conic, or the exact num/den
-// test against the record's cutoff for Gaussians without a usable conic)
-// must pass at every sample a kept Gaussian reaches. Input is a binary dump of one oracle
-// frame written by tests/test_cull.py; prints pair counts and the number of
-// violations of each (both must be 0), and the smallest relative slack
-// (det - q) / det of the conic test at a reached sample.
+// (cutoff, exp, alpha >= 1/255). Then, per pixel, the blend's first pass --
+// the staged Gaussian's sample mask (fh_mask_prep / fh_mask_row_w2 /
+// fh_mask_row_bits, every sample for records without a usable ellipse) --
+// must hold every sample a kept Gaussian reaches. The kernel takes the row
+// half-widths' square roots with v_sqrt_f32; the audit takes them lowered by
+// a relative 2^-20, far beyond that instruction's error, so its masks are
+// subsets of the kernel's. Input is a binary dump of one oracle frame written
+// by tests/test_cull.py; prints pair counts and the number of violations of
+// each (both must be 0), and the smallest slack, in pixels, between a reached
+// sample and its row interval's edge.
 //
 // With --waves it prints instead the blend's wave-level loop counts for the
 // frame, as the kernel runs it (batches of 64 staged Gaussians per 8x8 wave,
@@ -47,25 +50,29 @@ static bool reaches(const float* V1, const float* V2, const float* V4, float opa
     return fh_fmin(opacity * G, 1.0f) >= 1.0f / 255.0f;
 }
 
-// the blend's first-pass pre-test at pixel (px, py) of a tile centred at (ox, oy)
-static bool pre_test(const float* k, float cut, const float* V1, const float* V2, const float* V4, float pxf,
-                     float pyf, float ox, float oy, double* slack) {
-    if (k[5] >= 0.0f) {
-        const float u = pxf - ox, v = pyf - oy;
-        const float du = u - k[0], dv = v - k[1];
-        const float q = fmaf(k[2], du * du, fmaf(k[4], dv * dv, k[3] * (du * dv)));
-        if (slack) *slack = ((double)k[5] - (double)q) / (double)k[5];
-        return fh_pixel_in_conic(k, u, v);
+// The staged Gaussian's sample mask as the blend computes it for the samples
+// (bx + c step, by + r step), c, r < side, with each row's square root
+// lowered by a relative 2^-20 (see above). `all`: the record has no usable
+// ellipse (every sample). Per row, the interval's edges in columns go to lo
+// and hi (NaN for an empty row), for the slack report.
+static uint64_t sample_mask(const float rec[5], int bx, int by, int step, int side, bool* all, float lo[8],
+                            float hi[8], float uc_out[8]) {
+    const float inv_step = 1.0f / (float)step;
+    const FhMaskPrep p = fh_mask_prep(rec, 1.0f / rec[3], (float)bx, (float)by, inv_step);
+    *all = p.all;
+    if (p.all) return side == 8 ? ~0ull : 0xffffull;
+    uint64_t m = 0;
+    for (int r = 0; r < side; ++r) {
+        const float dy = fmaf((float)r, (float)step, p.dy0);
+        const float w2 = fh_mask_row_w2(p, dy);
+        const float w = w2 < 0.0f ? -1.0f : sqrtf(w2) * (1.0f - 0x1p-20f);
+        m |= (uint64_t)fh_mask_row_bits(p, dy, w, side) << (r * side);
+        const float uc = fmaf(p.s, dy, p.u0);
+        uc_out[r] = uc;
+        lo[r] = w < 0.0f ? NAN : (uc - w) - kMaskMarginPx;
+        hi[r] = w < 0.0f ? NAN : (uc + w) + kMaskMarginPx;
     }
-    float Px[4], Py[4];
-    for (int c = 0; c < 4; ++c) {
-        Px[c] = fmaf(-V4[c], pxf, V1[c]);
-        Py[c] = fmaf(-V4[c], pyf, V2[c]);
-    }
-    float m[3], d[3];
-    for (int c = 0; c < 3; ++c) m[c] = fmaf(Px[3], Py[c], -(Px[c] * Py[3]));
-    cross3(Px, Py, d);
-    return !(dot3(m, m) > cut * dot3(d, d));
+    return m;
 }
 
 template <class T>
@@ -99,9 +106,9 @@ static std::vector<std::pair<int, int>> samples(const Frame& F, int type, uint32
 // One lane group walking list [b, e) in batches of bs, culled against rect
 // `cull`, first-pass tested at `px` about the rect's centre: per batch the
 // kept count and each sample's count of passing Gaussians.
-static void walk(const Frame& F, const std::vector<std::pair<int, int>>& px, const float cull[4], uint32_t b,
-                 uint32_t e, uint32_t bs, std::vector<int>& kept, std::vector<std::vector<int>>& lane) {
-    const float ox = 0.5f * (cull[0] + cull[1]), oy = 0.5f * (cull[2] + cull[3]);
+static void walk(const Frame& F, const std::vector<std::pair<int, int>>& px, const float srect[4],
+                 const float cull[4], uint32_t b, uint32_t e, uint32_t bs, std::vector<int>& kept,
+                 std::vector<std::vector<int>>& lane) {
     for (uint32_t c = b; c < e; c = (e - c > bs) ? c + bs : e) {
         int k = 0;
         std::vector<int> cnt(px.size(), 0);
@@ -115,12 +122,14 @@ static void walk(const Frame& F, const std::vector<std::pair<int, int>>& px, con
             if (!fh_ellipse_may_touch(rec, 1.0f / rec[2], 1.0f / rec[3], cull[0], cull[1], cull[2], cull[3]))
                 continue;
             ++k;
-            float kq[6];
-            fh_ellipse_pixel_conic(rec, ox, oy, kq);
-            const float cut = fh_pixel_cutoff(o);
+            // the mask over this lane group's samples (rectangle srect)
+            const int side = px.size() == 16 ? 4 : 8;
+            const int step = (int)(srect[1] - srect[0]) / (side - 1);
+            bool all = false;
+            float lo[8], hi[8], uc[8];
+            const uint64_t m = sample_mask(rec, (int)srect[0], (int)srect[2], step, side, &all, lo, hi, uc);
             for (size_t q = 0; q < px.size(); ++q)
-                if (px[q].first >= 0)
-                    cnt[q] += pre_test(kq, cut, a1, a2, a4, (float)px[q].first, (float)px[q].second, ox, oy, nullptr);
+                if (px[q].first >= 0) cnt[q] += (int)((m >> q) & 1ull);
         }
         kept.push_back(k);
         lane.push_back(cnt);
@@ -161,7 +170,7 @@ static void wave_report(const Frame& F) {
                         float r[4];
                         const uint32_t tg = F.tmp[g + t];
                         const auto px = samples(F, type, tg, r);
-                        walk(F, px, r, F.ranges[2 * (size_t)tg], F.ranges[2 * (size_t)tg + 1], bs, kept[t], lane[t]);
+                        walk(F, px, r, r, F.ranges[2 * (size_t)tg], F.ranges[2 * (size_t)tg + 1], bs, kept[t], lane[t]);
                         nb = std::max(nb, kept[t].size());
                     }
                     for (size_t i = 0; i < nb; ++i) {
@@ -183,7 +192,7 @@ static void wave_report(const Frame& F) {
                 const auto px = samples(F, type, tg, r);
                 std::vector<int> kept;
                 std::vector<std::vector<int>> lane;
-                walk(F, px, r, F.ranges[2 * (size_t)tg], F.ranges[2 * (size_t)tg + 1], 64, kept, lane);
+                walk(F, px, r, r, F.ranges[2 * (size_t)tg], F.ranges[2 * (size_t)tg + 1], 64, kept, lane);
                 rounds += kept.size();
                 for (size_t i = 0; i < kept.size(); ++i) { p1 += kept[i]; p2 += max_of(lane[i]); }
                 alt[0] += merged(lane, 2);
@@ -201,7 +210,7 @@ static void wave_report(const Frame& F) {
                         const auto pxs = samples(F, type, lt * 4 + st, rs);
                         std::vector<int> k2;
                         std::vector<std::vector<int>> l2;
-                        walk(F, pxs, big, F.ranges[2 * (size_t)tg], F.ranges[2 * (size_t)tg + 1], 64, k2, l2);
+                        walk(F, pxs, rs, big, F.ranges[2 * (size_t)tg], F.ranges[2 * (size_t)tg + 1], 64, k2, l2);
                         if (st == 0) wt_rounds += k2.size();
                         for (int k : k2) wt_p1 += k;
                     }
@@ -270,25 +279,28 @@ int main(int argc, char** argv) {
                 kept += keep;
                 reach += any;
                 if (keep) {
-                    const float ox = 0.5f * (x0 + x1), oy = 0.5f * (y0 + y1);
-                    float k[6];
-                    fh_ellipse_pixel_conic(rec, ox, oy, k);
-                    no_conic += k[5] < 0.0f;
-                    const float cut = fh_pixel_cutoff(o);
+                    bool all = false;
+                    float lo[8], hi[8], uc[8];
+                    const uint64_t m = sample_mask(rec, bx, by, step, side, &all, lo, hi, uc);
+                    no_conic += all;
                     for (int q = 0; q < side * side; ++q) {
                         const int px = bx + (q % side) * step, py = by + (q / side) * step;
                         if (!(px < W && py < H)) continue;
-                        double sl = 1e30;
                         const bool r = reaches(a1, a2, a4, o, (float)px, (float)py);
-                        const bool pt = pre_test(k, cut, a1, a2, a4, (float)px, (float)py, ox, oy, &sl);
+                        const bool pt = (m >> q) & 1ull;
                         ++px_evals;
                         px_reach += r;
                         px_pre += pt;
-                        if (r && k[5] >= 0.0f && sl < min_slack) min_slack = sl;
+                        if (r && !all) {
+                            // slack in pixels between the sample's column and its row interval's edges
+                            const int c = q % side, rr = q / side;
+                            const double sl = std::min((double)c - (double)lo[rr], (double)hi[rr] - (double)c) * step;
+                            if (sl < min_slack) min_slack = sl;
+                        }
                         if (r && !pt) {
                             if (px_bad < 5)
-                                std::printf("pixel violation: tile %u prim %u px %d py %d k=(%a %a %a %a %a %a)\n", tg,
-                                            p, px, py, k[0], k[1], k[2], k[3], k[4], k[5]);
+                                std::printf("pixel violation: tile %u prim %u px %d py %d rec=(%a %a %a %a %a)\n", tg,
+                                            p, px, py, rec[0], rec[1], rec[2], rec[3], rec[4]);
                             ++px_bad;
                         }
                     }
