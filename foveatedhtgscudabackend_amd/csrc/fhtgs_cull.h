@@ -27,12 +27,11 @@
 //     rectangle grown by kCullMarginPx (centre inside, else the exact minimum
 //     of the convex quadratic over the rectangle's edges)? Dropping a
 //     Gaussian that reaches no sample of the tile leaves the image unchanged.
-//   * fh_ellipse_pixel_conic / fh_pixel_in_conic: the per-pixel test of the
-//     blend's first pass, in coordinates relative to the tile centre.
+//   * fh_mask_prep / fh_mask_row_w2 / fh_mask_row_bits: the staged
+//     Gaussian's sample mask, which the blend's first pass reads (below).
 // A record whose ellipse is not proper and finite is flagged: the tile test
-// then keeps the Gaussian and the pixel test, as for a too eccentric
-// ellipse, falls back to the reference's own num/den test against
-// fh_pixel_cutoff. Conservativeness is audited per (tile, Gaussian) pair and
+// then keeps the Gaussian and its mask, as for a too eccentric ellipse, holds
+// every sample. Conservativeness is audited per (tile, Gaussian) pair and
 // per pixel with these functions (tests/cull_audit.cpp).
 #pragma once
 #include "fhtgs_math.h"
@@ -44,22 +43,6 @@ constexpr float kPixelGrowPx = 0.25f;
 // centres beyond this many pixels from the origin are flagged (their
 // float32 position would lose more than 1/256 px)
 constexpr float kEllipseMaxCoord = 65536.0f;
-
-// Bound on rho^2 = num / den beyond which a pixel cannot pass the
-// reference's alpha test (inference.cuh:216-224) for this opacity:
-// alpha = min(o exp(-rho^2 / 2), 1) >= 1/255 needs rho^2 <= 2 ln(255 o). The
-// margin of 0.01 covers the rounding of the logarithm, the exponential, the
-// reciprocal and the products (a few ulp each, below 1e-5 in rho^2), so every
-// pair the reference keeps passes `!(num > bound * den)`. From an opacity of
-// about 0.995 the bound is the reference's own cutoff constant, and that
-// test is then the reference's cutoff test itself. NaN maps to the cutoff
-// constant and -inf to 0 (the selects, unlike min/max, are explicit about
-// NaN), which keeps every pair whose num or den is NaN.
-FH_HD float fh_pixel_cutoff(float opacity) {
-    float c = 2.0f * fh_logf(255.0f * opacity) + 0.01f;
-    c = c < 11.0825270903f ? c : 11.0825270903f;
-    return c > 0.0f ? c : 0.0f;
-}
 
 // The ellipse record, 5 floats, stored as a float4 and a float:
 //   r[0], r[1]  centre (cx, cy) in pixel coordinates
@@ -138,29 +121,6 @@ FH_HD bool fh_ellipse_may_touch(const float r[5], float inv11, float inv22, floa
     best = fh_fmin(best, q(fh_clamp(rv * v0, u0, u1), v0));
     best = fh_fmin(best, q(fh_clamp(rv * v1, u0, u1), v1));
     return best <= det;
-}
-
-// Pixel conic of the first pass relative to the tile centre (ox, oy):
-// k[0..5] = ellipse centre (cx, cy), the quadratic form q(du, dv) =
-// k2 du^2 + k3 du dv + k4 dv^2 (k2 = M22, k3 = -2 M12, k4 = M11) and its bound
-// k5 = det M; a pixel at offset (u, v) passes when q(u - cx, v - cy) <= k5.
-// k5 = -1 marks a Gaussian without a usable ellipse (flagged, or so eccentric
-// that det M <= 1e-4 m11 m22): the first pass then uses the exact test.
-FH_HD void fh_ellipse_pixel_conic(const float r[5], float ox, float oy, float k[6]) {
-    const float m11 = r[2], m22 = r[3], m12 = r[4];
-    const float det = fmaf(m11, m22, -(m12 * m12));
-    const bool ok = m11 > 0.0f && det > 1e-4f * (m11 * m22);
-    k[0] = ok ? r[0] - ox : 0.0f;
-    k[1] = ok ? r[1] - oy : 0.0f;
-    k[2] = ok ? m22 : 0.0f;
-    k[3] = ok ? -2.0f * m12 : 0.0f;
-    k[4] = ok ? m11 : 0.0f;
-    k[5] = ok ? det : -1.0f;
-}
-
-FH_HD bool fh_pixel_in_conic(const float k[6], float u, float v) {
-    const float du = u - k[0], dv = v - k[1];
-    return fmaf(k[2], du * du, fmaf(k[4], dv * dv, k[3] * (du * dv))) <= k[5];
 }
 
 // ---- sample masks: which samples of a tile an ellipse record can reach

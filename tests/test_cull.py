@@ -1,9 +1,10 @@
 """The blend's per-tile cull (fh_ellipse_may_touch in csrc/fhtgs_cull.h, on the
 ellipse record fh_ellipse_record that the preprocess stores per Gaussian) and
-its per-pixel first-pass pre-test (fh_ellipse_pixel_conic / fh_pixel_in_conic)
-are optimisations that must not change the image: every Gaussian the cull drops
-from a tile must reach no sample of that tile, and the pre-test must keep
-every sample a kept Gaussian reaches. tests/cull_audit.cpp checks both over
+its first pass over each staged Gaussian's sample mask (fh_mask_prep /
+fh_mask_row_bits, row intervals of the ellipse) are optimisations that must not
+change the image: every Gaussian the cull drops from a tile must reach no
+sample of that tile, and the mask must hold every sample a kept Gaussian
+reaches. tests/cull_audit.cpp checks both over
 every (tile, instance) pair and sample of oracle frames, with the blend's own
 float32 operations, and reports how tight they are."""
 import os
