@@ -241,8 +241,7 @@ void carve_fixed(void* base, int64_t N, int n_large, int n_small, ViewBuffers& v
     vb.n_touched = c.take<uint32_t>((size_t)N);
     vb.bounds = c.take<ushort4>((size_t)N);
     vb.prim = c.take<float4>((size_t)N * 4);
-    vb.ell_a = c.take<float4>((size_t)N);
-    vb.ell_b = c.take<float>((size_t)N);
+    vb.ell = c.take<float4>((size_t)N * 2);
     vb.block_sums = c.take<uint32_t>(std::max<size_t>(nb, 1));
     vb.tab_mask = vb.tab_vis_sat = vb.tab_fovea_sat = nullptr;
     if (R >= 0) {

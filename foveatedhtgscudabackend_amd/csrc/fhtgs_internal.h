@@ -110,8 +110,8 @@ struct ViewBuffers {
     uint32_t* n_touched;      // N
     ushort4* bounds;          // N
     float4* prim;             // 4N: one 64-B record per primitive (VPMT1, VPMT2, VPMT4, rgba)
-    float4* ell_a;            // N: support-ellipse record (fh_ellipse_record): centre, m11, m22
-    float* ell_b;             // N: its m12
+    float4* ell;              // 2N: support-ellipse record (fh_ellipse_record), 32 B per Gaussian:
+                              // (cx, cy, m11, m22), (m12, 0, 0, 0) -- one memory sector per gather
     uint32_t* block_sums;     // ceil(N / kPrimBlock), then exclusive offsets
     // culling tables built on the device when the view brings none
     uint32_t* tab_mask;       // ceil(n_large / 32)

@@ -125,8 +125,8 @@ struct BlendArgs {
     const uint2* ranges;
     const uint32_t* vals;
     const float4* prim;  // 4 float4 per primitive: VPMT1, VPMT2, VPMT4, rgba
-    const float4* ell_a;  // per primitive: support-ellipse record (fh_ellipse_record): centre, m11, m22
-    const float* ell_b;   // per primitive: its m12
+    const float4* ell;  // per primitive, 32 B: support-ellipse record (fh_ellipse_record):
+                        // (cx, cy, m11, m22), (m12, 0, 0, 0)
     uint32_t xcd_run;
 };
 
@@ -214,12 +214,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 5 :
         float inv22 = 0.0f;
         if (idx < range.y) {
             const uint32_t p = a.vals[idx];
-            const float4 E = a.ell_a[p];
+            const float4 E = a.ell[2 * (size_t)p];
             er[0] = E.x;
             er[1] = E.y;
             er[2] = E.z;
             er[3] = E.w;
-            er[4] = a.ell_b[p];
+            er[4] = a.ell[2 * (size_t)p + 1].x;
             const float4* rec = a.prim + 4 * (size_t)p;
             V1 = rec[0];
             V2 = rec[1];
@@ -524,8 +524,7 @@ hipError_t launch_blend(const ViewParams* dvp, const ViewBuffers& vb, const uint
     a.ranges = vb.ranges;
     a.vals = sorted_vals;
     a.prim = vb.prim;
-    a.ell_a = vb.ell_a;
-    a.ell_b = vb.ell_b;
+    a.ell = vb.ell;
     a.xcd_run = (uint32_t)(xcd_run > 0 ? xcd_run : 0);
     // rasterization/src/inference.cu:17-31: largest power of two <= K in 1..32
     if (K >= 32) blend_kernel<32><<<n_active, 64, 0, s>>>(dvp, a);

@@ -117,7 +117,8 @@ __device__ __forceinline__ v4 frame_row(v4 row, v3 u, v3 v, v3 w, v3 mu) {
 }
 
 // the view's record row and support-ellipse record of a visible Gaussian
-// (both ellipse arrays are written with unit stride across the lanes). The
+// (the ellipse as 32 B, so that the blend gathers it in one memory sector;
+// 20 B in two arrays cost the blend a second sector per listed Gaussian). The
 // logarithm is taken again here rather than kept in a register through the
 // SH gathers, which need every register they can get.
 __device__ __forceinline__ void store_record(const ViewBuffers& vb, uint32_t i, v4 V1, v4 V2, v4 V4,
@@ -132,8 +133,8 @@ __device__ __forceinline__ void store_record(const ViewBuffers& vb, uint32_t i, 
     const float a4[4] = {V4.x, V4.y, V4.z, V4.w};
     float r[5];
     fh_ellipse_record(a1, a2, a4, c2, r);
-    vb.ell_a[i] = make_float4(r[0], r[1], r[2], r[3]);
-    vb.ell_b[i] = r[4];
+    vb.ell[2 * (size_t)i] = make_float4(r[0], r[1], r[2], r[3]);
+    vb.ell[2 * (size_t)i + 1] = make_float4(r[4], 0.0f, 0.0f, 0.0f);
 }
 
 __device__ __forceinline__ uint32_t tile_lo(float c, float e, int grid) {
