@@ -23,20 +23,18 @@
 // matrix grown by kPixelGrowPx^2 I -- a superset of every pixel the
 // reference's cutoff and alpha tests keep, up to rounding. The blend then
 // only translates it per tile:
-//   * fh_ellipse_may_touch: does the ellipse meet the tile's sample
-//     rectangle grown by kCullMarginPx (centre inside, else the exact minimum
-//     of the convex quadratic over the rectangle's edges)? Dropping a
-//     Gaussian that reaches no sample of the tile leaves the image unchanged.
-//   * fh_mask_prep / fh_mask_row_w2 / fh_mask_row_bits: the staged
-//     Gaussian's sample mask, which the blend's first pass reads (below).
-// A record whose ellipse is not proper and finite is flagged: the tile test
-// then keeps the Gaussian and its mask, as for a too eccentric ellipse, holds
-// every sample. Conservativeness is audited per (tile, Gaussian) pair and
-// per pixel with these functions (tests/cull_audit.cpp).
+//   * fh_mask_prep / fh_mask_row_w2 / fh_mask_row_bits: the listed
+//     Gaussian's sample mask, which samples of the tile its ellipse can
+//     contain (below). The blend drops a Gaussian whose mask is empty (it
+//     reaches no sample of the tile, so the image is unchanged) and its
+//     pixels' first pass reads one bit per staged Gaussian.
+// A record whose ellipse is not proper and finite is flagged: its mask, as
+// for a too eccentric ellipse, holds every sample. Conservativeness is
+// audited per (tile, Gaussian) pair and per sample with these functions
+// (tests/cull_audit.cpp).
 #pragma once
 #include "fhtgs_math.h"
 
-constexpr float kCullMarginPx = 0.0625f;
 constexpr float kPixelLevelRel = 0.02f;
 constexpr float kPixelLevelAbs = 0.02f;
 constexpr float kPixelGrowPx = 0.25f;
@@ -95,34 +93,6 @@ FH_HD void fh_ellipse_record(const float a1[4], const float a2[4], const float a
     r[4] = m12;
 }
 
-// Tile test: keep the Gaussian iff its record's ellipse meets the rectangle
-// [x0, x1] x [y0, y1] of the tile's sample positions grown by kCullMarginPx
-// (or the record is flagged unusable). inv11, inv22 are 1 / m11 and 1 / m22,
-// correctly rounded (the kernels use a 3-instruction reciprocal that equals
-// IEEE division on every float).
-FH_HD bool fh_ellipse_may_touch(const float r[5], float inv11, float inv22, float x0, float x1, float y0,
-                                float y1) {
-    const float m11 = r[2], m22 = r[3], m12 = r[4];
-    if (!(m11 > 0.0f)) return true;
-    // coordinates relative to the rectangle centre
-    const float ox = 0.5f * (x0 + x1), oy = 0.5f * (y0 + y1);
-    const float hx = 0.5f * (x1 - x0) + kCullMarginPx, hy = 0.5f * (y1 - y0) + kCullMarginPx;
-    const float cx = r[0] - ox, cy = r[1] - oy;
-    if (cx >= -hx && cx <= hx && cy >= -hy && cy <= hy) return true;
-    const float det = fmaf(m11, m22, -(m12 * m12));
-    // q(u, v) = m22 u^2 - 2 m12 u v + m11 v^2 <= det  <=>  inside the ellipse
-    auto q = [&](float u, float v) { return fmaf(m22, u * u, fmaf(m11, v * v, -2.0f * (m12 * (u * v)))); };
-    const float u0 = -hx - cx, u1 = hx - cx, v0 = -hy - cy, v1 = hy - cy;
-    const float ru = m12 * inv11, rv = m12 * inv22;
-    // edges x = const: the minimum over v is at v = (m12 / m11) u
-    float best = q(u0, fh_clamp(ru * u0, v0, v1));
-    best = fh_fmin(best, q(u1, fh_clamp(ru * u1, v0, v1)));
-    // edges y = const: the minimum over u is at u = (m12 / m22) v
-    best = fh_fmin(best, q(fh_clamp(rv * v0, u0, u1), v0));
-    best = fh_fmin(best, q(fh_clamp(rv * v1, u0, u1), v1));
-    return best <= det;
-}
-
 // ---- sample masks: which samples of a tile an ellipse record can reach
 //
 // The samples of a tile are (bx + c step, by + r step), c < cols, r < rows
@@ -131,14 +101,19 @@ FH_HD bool fh_ellipse_may_touch(const float r[5], float inv11, float inv22, floa
 //     x = x_c(y) +- w(y),  x_c(y) = cx + (m12 / m22)(y - cy),
 //     w(y)^2 = det / m22^2 * (m22 - (y - cy)^2),
 // so the row's bits are the columns of [x_c - w, x_c + w] widened by
-// kMaskMarginPx on each side. The blend's staging lane computes one mask per
-// staged Gaussian (the square root with the hardware's v_sqrt_f32); the
-// pixels' first pass then reads one bit per staged Gaussian instead of
-// testing a conic. Records that are flagged, or too eccentric for det to be
-// accurate (det <= 1e-4 m11 m22, the same rule as the pixel conic), get
-// every sample. tests/cull_audit.cpp checks, with the square root lowered
-// by a relative 2^-20, that the masks keep every sample the reference's
-// cutoff and alpha tests keep.
+// kMaskMarginPx on each side. The blend's staging lane computes the mask of
+// each listed Gaussian (the square root with the hardware's v_sqrt_f32) and
+// stages it only if the mask holds a sample -- that is the blend's tile cull
+// -- and the pixels' first pass then reads one bit per staged Gaussian.
+// Records that are flagged, or too eccentric for det to be accurate
+// (det <= 1e-4 m11 m22), get every sample. tests/cull_audit.cpp checks, with
+// the square root lowered by a relative 2^-20, that the masks keep every
+// sample the reference's cutoff and alpha tests keep.
+//
+// For a usable record every quantity below is finite: |cx|, |cy| < 65536,
+// m11, m22 >= 1/16 (the growth) and < 1e17, so s < 2e18 and a <= m11 / m22 <
+// 2e18; a * (m22 - dy^2) may round to -inf or +inf, which the clamps below
+// turn into an empty or a full row.
 constexpr float kMaskMarginPx = 0.015625f;
 
 struct FhMaskPrep {
@@ -170,17 +145,26 @@ FH_HD float fh_mask_row_w2(const FhMaskPrep& p, float row_dy) {
     return p.a * fmaf(-row_dy, row_dy, p.m22);
 }
 
+// clamp(x, lo, hi) for lo <= hi and x not NaN: v_med3_f32 on the GPU, the
+// same value as max(lo, min(x, hi)) on the host.
+FH_HD float fh_med3(float x, float lo, float hi) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_fmed3f(x, lo, hi);
+#else
+    return fh_fmax(lo, fh_fmin(x, hi));
+#endif
+}
+
 // The column bits (at most 8 columns) of the row at y offset row_dy from the
-// centre, given w, the square root of fh_mask_row_w2 in columns; the caller
-// passes -1 when that is negative (the row misses). A NaN w or NaN bound
-// gives the whole row: fmax / fmin return their other operand.
+// centre, given the row's half width in columns already widened by the
+// margin (sqrt(fh_mask_row_w2) + kMaskMarginPx), or -1 when fh_mask_row_w2
+// is negative (the row misses: the columns then form an empty range).
 FH_HD uint32_t fh_mask_row_bits(const FhMaskPrep& p, float row_dy, float w, int cols) {
     const float uc = fmaf(p.s, row_dy, p.u0);
-    const float lo = fh_fmax(ceilf((uc - w) - kMaskMarginPx), 0.0f);
-    const float hi = fh_fmin(floorf((uc + w) + kMaskMarginPx), (float)(cols - 1));
-    // both converted inside [0, cols - 1]; the bits count only if lo <= hi
-    const uint32_t il = (uint32_t)fh_fmin(lo, (float)(cols - 1));
-    const uint32_t ih = (uint32_t)fh_fmax(hi, 0.0f);
-    const uint32_t bits = (2u << ih) - (1u << il);
-    return (!(w < 0.0f) && !(lo > hi)) ? bits : 0u;
+    const float fc = (float)cols;
+    // columns [b, e): b = ceil(uc - w), e = floor(uc + w) + 1, both in [0, cols]
+    const float b = fh_med3(ceilf(uc - w), 0.0f, fc);
+    const float e = fh_med3(floorf(uc + w) + 1.0f, 0.0f, fc);
+    const uint32_t ib = (uint32_t)b, n = (uint32_t)fh_fmax(e - b, 0.0f);
+    return ((1u << n) - 1u) << ib;
 }

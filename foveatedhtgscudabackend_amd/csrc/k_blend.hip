@@ -187,8 +187,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 5 :
     const uint32_t py = by + (li / cols) * step;
     const bool inside = have && px < (uint32_t)vp.width && py < (uint32_t)vp.height;
     const float pxf = (float)px, pyf = (float)py;
-    const float tx0 = (float)bx, tx1 = (float)(bx + (cols - 1u) * step);
-    const float ty0 = (float)by, ty1 = (float)(by + (cols - 1u) * step);
+    const float tx0 = (float)bx, ty0 = (float)by;  // the group's first sample
     const float stepf = (float)step, inv_step = quarter ? 0.25f : (step == 2u ? 0.5f : 1.0f);
     // this lane's bit in its group's sample masks (first pass)
     const uint32_t m_word = li >> 5, m_shift = li & 31u;
@@ -203,31 +202,44 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 5 :
 
     const uint2 range = have ? a.ranges[tg] : make_uint2(0u, 0u);
     for (uint32_t b0 = range.x; __ballot(b0 < range.y) != 0ull; b0 += gsz) {
-        // each lane loads one instance of its group's next gsz and tests its
-        // support ellipse (the preprocess's record) against the tile's
-        // samples (fh_ellipse_may_touch); survivors are compacted in order
-        // into the group's part of the LDS batch
+        // each lane loads one instance of its group's next gsz and computes
+        // its sample mask from the support ellipse (the preprocess's record):
+        // which of the group's samples the ellipse can reach, row by row
+        // (fh_mask_row_bits; every sample for a record without a usable
+        // ellipse). A Gaussian whose mask holds no sample reaches none and is
+        // dropped (the tile cull); the others are compacted in order into the
+        // group's part of the LDS batch with their masks.
         const uint32_t idx = b0 + li;
-        bool keep = false;
         float4 V1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), V2 = V1, V4 = V1, C = V1;
-        float er[5] = {0.0f, 0.0f, -1.0f, 0.0f, 0.0f};
-        float inv22 = 0.0f;
+        uint32_t m0 = 0u, m1 = 0u;
         if (idx < range.y) {
             const uint32_t p = a.vals[idx];
             const float4 E = a.ell[2 * (size_t)p];
-            er[0] = E.x;
-            er[1] = E.y;
-            er[2] = E.z;
-            er[3] = E.w;
-            er[4] = a.ell[2 * (size_t)p + 1].x;
+            const float er[5] = {E.x, E.y, E.z, E.w, a.ell[2 * (size_t)p + 1].x};
             const float4* rec = a.prim + 4 * (size_t)p;
             V1 = rec[0];
             V2 = rec[1];
             V4 = rec[2];
             C = rec[3];
-            inv22 = rcp_rn(er[3]);
-            keep = fh_ellipse_may_touch(er, rcp_rn(er[2]), inv22, tx0, tx1, ty0, ty1);
+            const FhMaskPrep mp = fh_mask_prep(er, rcp_rn(er[3]), tx0, ty0, inv_step);
+            if (mp.all) {
+                m0 = quarter ? 0xffffu : ~0u;
+                m1 = quarter ? 0u : ~0u;
+            } else {
+#pragma unroll
+                for (uint32_t r = 0; r < 8u; ++r) {
+                    if (r >= cols) continue;
+                    const float dy = fmaf((float)r, stepf, mp.dy0);
+                    const float w2 = fh_mask_row_w2(mp, dy);
+                    const float w = w2 < 0.0f ? -1.0f : __builtin_amdgcn_sqrtf(w2) + kMaskMarginPx;
+                    const uint32_t bits = fh_mask_row_bits(mp, dy, w, (int)cols);
+                    const uint32_t sh = r * cols;
+                    if (sh < 32u) m0 |= bits << sh;
+                    else m1 |= bits << (sh - 32u);
+                }
+            }
         }
+        const bool keep = (m0 | m1) != 0u;
         const uint64_t km = __ballot(keep);
         if (km == 0ull) continue;
         const int count = (int)__popcll(km & gmask);
@@ -246,28 +258,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 5 :
             sV2[pos] = V2;
             sV4[pos] = V4;
             sC[pos] = C;
-            // first-pass data: which of the group's samples the support
-            // ellipse can reach, row by row (fh_mask_row_bits; every sample
-            // for a record without a usable ellipse, which the second pass's
-            // own tests then settle)
-            const FhMaskPrep mp = fh_mask_prep(er, inv22, tx0, ty0, inv_step);
-            uint32_t m0 = ~0u, m1 = quarter ? 0u : ~0u;
-            if (quarter) m0 = 0xffffu;
-            if (!mp.all) {
-                m0 = 0u;
-                m1 = 0u;
-#pragma unroll
-                for (uint32_t r = 0; r < 8u; ++r) {
-                    if (r >= cols) continue;
-                    const float dy = fmaf((float)r, stepf, mp.dy0);
-                    const float w2 = fh_mask_row_w2(mp, dy);
-                    const float w = w2 < 0.0f ? -1.0f : __builtin_amdgcn_sqrtf(w2);
-                    const uint32_t bits = fh_mask_row_bits(mp, dy, w, (int)cols);
-                    const uint32_t sh = r * cols;
-                    if (sh < 32u) m0 |= bits << sh;
-                    else m1 |= bits << (sh - 32u);
-                }
-            }
             sM[2u * pos] = m0;
             sM[2u * pos + 1u] = m1;
         }

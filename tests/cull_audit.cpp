@@ -1,14 +1,15 @@
-// Audit of the blend's per-tile cull (fh_ellipse_may_touch on the ellipse
-// record fh_ellipse_record that the preprocess stores): over every (tile,
-// instance) pair of a frame, a Gaussian that the cull drops must reach no
-// sample of the tile, evaluated with the blend's own float32 operations
-// (cutoff, exp, alpha >= 1/255). Then, per pixel, the blend's first pass --
-// the staged Gaussian's sample mask (fh_mask_prep / fh_mask_row_w2 /
-// fh_mask_row_bits, every sample for records without a usable ellipse) --
-// must hold every sample a kept Gaussian reaches. The kernel takes the row
-// half-widths' square roots with v_sqrt_f32; the audit takes them lowered by
-// a relative 2^-20, far beyond that instruction's error, so its masks are
-// subsets of the kernel's. Input is a binary dump of one oracle frame written
+// Audit of the blend's sample masks (fh_mask_prep / fh_mask_row_w2 /
+// fh_mask_row_bits on the ellipse record fh_ellipse_record that the
+// preprocess stores; every sample for records without a usable ellipse): the
+// blend drops a listed Gaussian whose mask holds no sample of the tile (the
+// tile cull), and its pixels evaluate only the staged Gaussians whose masks
+// hold their bit (the first pass). Over every (tile, instance) pair of a
+// frame and every sample, with the blend's own float32 operations (cutoff,
+// exp, alpha >= 1/255), a sample a Gaussian reaches must be in its mask: then
+// a dropped Gaussian reaches no sample and a pixel skips no Gaussian that
+// reaches it. The kernel takes the row half-widths' square roots with
+// v_sqrt_f32; the audit takes them lowered by a relative 2^-20, far beyond
+// that instruction's error, so its masks are subsets of the kernel's. Input is a binary dump of one oracle frame written
 // by tests/test_cull.py; prints pair counts and the number of violations of
 // each (both must be 0), and the smallest slack, in pixels, between a reached
 // sample and its row interval's edge.
@@ -65,12 +66,12 @@ static uint64_t sample_mask(const float rec[5], int bx, int by, int step, int si
     for (int r = 0; r < side; ++r) {
         const float dy = fmaf((float)r, (float)step, p.dy0);
         const float w2 = fh_mask_row_w2(p, dy);
-        const float w = w2 < 0.0f ? -1.0f : sqrtf(w2) * (1.0f - 0x1p-20f);
+        const float w = w2 < 0.0f ? -1.0f : sqrtf(w2) * (1.0f - 0x1p-20f) + kMaskMarginPx;
         m |= (uint64_t)fh_mask_row_bits(p, dy, w, side) << (r * side);
         const float uc = fmaf(p.s, dy, p.u0);
         uc_out[r] = uc;
-        lo[r] = w < 0.0f ? NAN : (uc - w) - kMaskMarginPx;
-        hi[r] = w < 0.0f ? NAN : (uc + w) + kMaskMarginPx;
+        lo[r] = w < 0.0f ? NAN : uc - w;
+        hi[r] = w < 0.0f ? NAN : uc + w;
     }
     return m;
 }
@@ -119,15 +120,25 @@ static void walk(const Frame& F, const std::vector<std::pair<int, int>>& px, con
             const float o = F.rgba[4 * (size_t)p + 3];
             float rec[5];
             fh_ellipse_record(a1, a2, a4, 2.0f * fh_logf(o * 255.0f), rec);
-            if (!fh_ellipse_may_touch(rec, 1.0f / rec[2], 1.0f / rec[3], cull[0], cull[1], cull[2], cull[3]))
-                continue;
-            ++k;
-            // the mask over this lane group's samples (rectangle srect)
+            // the mask over this lane group's samples (rectangle srect); the
+            // tile cull keeps a Gaussian whose mask over the cull rectangle's
+            // samples holds one (the same samples unless a 16x16 work tile
+            // is estimated)
             const int side = px.size() == 16 ? 4 : 8;
             const int step = (int)(srect[1] - srect[0]) / (side - 1);
             bool all = false;
             float lo[8], hi[8], uc[8];
             const uint64_t m = sample_mask(rec, (int)srect[0], (int)srect[2], step, side, &all, lo, hi, uc);
+            bool kept = m != 0;
+            if (cull != srect) {
+                kept = false;
+                for (int sy = 0; sy < 2 && !kept; ++sy)
+                    for (int sx = 0; sx < 2 && !kept; ++sx)
+                        kept = sample_mask(rec, (int)cull[0] + 8 * sx, (int)cull[2] + 8 * sy, 1, 8, &all, lo, hi,
+                                           uc) != 0;
+            }
+            if (!kept) continue;
+            ++k;
             for (size_t q = 0; q < px.size(); ++q)
                 if (px[q].first >= 0) cnt[q] += (int)((m >> q) & 1ull);
         }
@@ -261,15 +272,16 @@ int main(int argc, char** argv) {
             const uint32_t tg = tmp[g], lt = tg / 4, st = tg % 4;
             const int bx = (int)(lt % gw) * 16 + (int)(st % 2) * 8;
             const int by = (int)(lt / gw) * 16 + (int)(st / 2) * 8;
-            const float x0 = (float)bx, x1 = (float)(bx + (side - 1) * step);
-            const float y0 = (float)by, y1 = (float)(by + (side - 1) * step);
             for (uint32_t j = ranges[2 * (size_t)tg]; j < ranges[2 * (size_t)tg + 1]; ++j) {
                 const uint32_t p = vals[j];
                 const float *a1 = &v1[4 * (size_t)p], *a2 = &v2[4 * (size_t)p], *a4 = &v4[4 * (size_t)p];
                 const float o = rgba[4 * (size_t)p + 3];
                 float rec[5];
                 fh_ellipse_record(a1, a2, a4, 2.0f * fh_logf(o * 255.0f), rec);
-                const bool keep = fh_ellipse_may_touch(rec, 1.0f / rec[2], 1.0f / rec[3], x0, x1, y0, y1);
+                bool all = false;
+                float lo[8], hi[8], uc[8];
+                const uint64_t m = sample_mask(rec, bx, by, step, side, &all, lo, hi, uc);
+                const bool keep = m != 0;
                 bool any = false;
                 for (int q = 0; q < side * side && !any; ++q) {
                     const int px = bx + (q % side) * step, py = by + (q / side) * step;
@@ -279,9 +291,6 @@ int main(int argc, char** argv) {
                 kept += keep;
                 reach += any;
                 if (keep) {
-                    bool all = false;
-                    float lo[8], hi[8], uc[8];
-                    const uint64_t m = sample_mask(rec, bx, by, step, side, &all, lo, hi, uc);
                     no_conic += all;
                     for (int q = 0; q < side * side; ++q) {
                         const int px = bx + (q % side) * step, py = by + (q / side) * step;

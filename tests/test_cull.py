@@ -1,12 +1,12 @@
-"""The blend's per-tile cull (fh_ellipse_may_touch in csrc/fhtgs_cull.h, on the
-ellipse record fh_ellipse_record that the preprocess stores per Gaussian) and
-its first pass over each staged Gaussian's sample mask (fh_mask_prep /
-fh_mask_row_bits, row intervals of the ellipse) are optimisations that must not
-change the image: every Gaussian the cull drops from a tile must reach no
-sample of that tile, and the mask must hold every sample a kept Gaussian
-reaches. tests/cull_audit.cpp checks both over
-every (tile, instance) pair and sample of oracle frames, with the blend's own
-float32 operations, and reports how tight they are."""
+"""The blend's sample masks (fh_mask_prep / fh_mask_row_bits in
+csrc/fhtgs_cull.h, row intervals of the ellipse record fh_ellipse_record that
+the preprocess stores per Gaussian) are its tile cull -- a listed Gaussian
+whose mask holds no sample of the tile is dropped -- and its first pass -- a
+pixel evaluates only the staged Gaussians whose masks hold its bit. They must
+not change the image: every sample a Gaussian reaches must be in its mask.
+tests/cull_audit.cpp checks that over every (tile, instance) pair and sample
+of oracle frames, with the blend's own float32 operations, and reports how
+tight the masks are."""
 import os
 import re
 import subprocess
