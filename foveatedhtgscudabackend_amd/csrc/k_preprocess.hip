@@ -17,7 +17,9 @@
 namespace fhtgs {
 
 constexpr int kMaxViewsPerPass = 2;
-constexpr int kShChunk = 24;  // visible Gaussians per SH gather round (8, 16, 32 measured slower)
+// LDS slots of SH coefficients per wave (a visible Gaussian each; a wave
+// with more visible Gaussians copies the rest in a further round)
+constexpr int kShSlots = 36;
 
 // Kernel argument: the views' parameters and the scene's tensors by value
 // (pointers read from kernel arguments are global ones to the compiler). The
@@ -409,7 +411,7 @@ __global__ __launch_bounds__(kPrimBlock) void preprocess_aaa_kernel(PreprocessAr
 template <int NV>
 __global__ __launch_bounds__(kPrimBlock) void preprocess_kernel(PreprocessArgs g) {
     const PreprocessState a = resolve<NV>(g);
-    extern __shared__ float s_sh[];  // per wave: 64 slots of the first min(3S, 45) SH floats
+    extern __shared__ float s_sh[];  // per wave: kShSlots slots of the first min(3S, 45) SH floats
     __shared__ uint32_t s_sum[NV][kPrimBlock / 64];
     __shared__ uint32_t s_vis[NV][kPrimBlock / 64];
     const uint32_t i = blockIdx.x * kPrimBlock + threadIdx.x;
@@ -418,7 +420,7 @@ __global__ __launch_bounds__(kPrimBlock) void preprocess_kernel(PreprocessArgs g
 
     uint32_t touched[NV];
     bool visible[NV];
-    // per view: the record rows and bounds, stored after the SH loads (one
+    // per view: the record rows and bounds, stored after every load (one
     // counter tracks loads and stores, so loads after stores wait for them)
     v4 R1[NV], R2[NV], R4[NV];
     ushort4 bnd[NV];
@@ -426,14 +428,21 @@ __global__ __launch_bounds__(kPrimBlock) void preprocess_kernel(PreprocessArgs g
 #pragma unroll
     for (int v = 0; v < NV; ++v) { touched[v] = 0u; visible[v] = false; }
     v3 mu{0.0f, 0.0f, 0.0f};
-    float opacity = 0.0f;
-
+    float opacity = 0.0f, s0[3] = {0.0f, 0.0f, 0.0f};
+    bool zok[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) zok[v] = false;
+    v3 sc{0.0f, 0.0f, 0.0f};
+    float R11 = 0.0f, R12 = 0.0f, R13 = 0.0f, R21 = 0.0f, R22 = 0.0f, R23 = 0.0f, R31 = 0.0f, R32 = 0.0f,
+          R33 = 0.0f, rho = 0.0f;
     if (valid) {
         mu = v3{a.pos[3 * i + 0], a.pos[3 * i + 1], a.pos[3 * i + 2]};
         opacity = a.opa[i];
+        s0[0] = a.sh0[3 * (size_t)i + 0];
+        s0[1] = a.sh0[3 * (size_t)i + 1];
+        s0[2] = a.sh0[3 * (size_t)i + 2];
         // view depth cull first (kernel_utils.cuh:121-123)
         bool any = false;
-        bool zok[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             const v4 M3 = load_row(a.vp[v].M, 2);
@@ -442,125 +451,135 @@ __global__ __launch_bounds__(kPrimBlock) void preprocess_kernel(PreprocessArgs g
             any |= zok[v];
         }
         if (any) {
-            const v3 sc{a.scl[3 * i + 0], a.scl[3 * i + 1], a.scl[3 * i + 2]};
+            sc = v3{a.scl[3 * i + 0], a.scl[3 * i + 1], a.scl[3 * i + 2]};
             const float4 q = *reinterpret_cast<const float4*>(a.rot + 4 * (size_t)i);
             // kernel_utils.cuh:76-88, quaternion (r, x, y, z)
             const float r = q.x, x = q.y, y = q.z, z = q.w;
             const float xx = x * x, yy = y * y, zz = z * z;
             const float xy = x * y, xz = x * z, yz = y * z;
             const float rx = r * x, ry = r * y, rz = r * z;
-            const float R11 = fmaf(-2.0f, yy + zz, 1.0f), R12 = 2.0f * (xy - rz), R13 = 2.0f * (xz + ry);
-            const float R21 = 2.0f * (xy + rz), R22 = fmaf(-2.0f, xx + zz, 1.0f), R23 = 2.0f * (yz - rx);
-            const float R31 = 2.0f * (xz - ry), R32 = 2.0f * (yz + rx), R33 = fmaf(-2.0f, xx + yy, 1.0f);
-            const float rho = 2.0f * fh_logf(opacity * kMinAlphaRcp);
-#pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                if (!zok[v]) continue;
-                const ViewParams& vp = a.vp[v];
-                const float sm = vp.scale_mod;
-                const v3 u{(R11 * sc.x) * sm, (R21 * sc.x) * sm, (R31 * sc.x) * sm};
-                const v3 vv{(R12 * sc.y) * sm, (R22 * sc.y) * sm, (R32 * sc.y) * sm};
-                const v3 w{(R13 * sc.z) * sm, (R23 * sc.z) * sm, (R33 * sc.z) * sm};
-                const v4 V4 = frame_row(load_row(vp.VPM, 3), u, vv, w, mu);
-                const v4 d{rho, rho, rho, -1.0f};
-                const float s = dot4(d, mul4(V4, V4));
-                if (s == 0.0f) continue;
-                const float rs = 1.0f / s;
-                const v4 f{rs * d.x, rs * d.y, rs * d.z, rs * d.w};
-                const v4 V3 = frame_row(load_row(vp.VPM, 2), u, vv, w, mu);
-                const float cz = dot4(f, mul4(V3, V4));
-                const float ez = sqrtf(fmaxf(fmaf(cz, cz, -dot4(f, mul4(V3, V3))), 0.0f));
-                if (cz - ez < -1.0f || cz + ez > 1.0f) continue;
-                const v4 V1 = frame_row(load_row(vp.VPM, 0), u, vv, w, mu);
-                const float cx = dot4(f, mul4(V1, V4));
-                const float ex = sqrtf(fmaxf(fmaf(cx, cx, -dot4(f, mul4(V1, V1))), 0.0f));
-                const v4 V2 = frame_row(load_row(vp.VPM, 1), u, vv, w, mu);
-                const float cy = dot4(f, mul4(V2, V4));
-                const float ey = sqrtf(fmaxf(fmaf(cy, cy, -dot4(f, mul4(V2, V2))), 0.0f));
-                const uint32_t b0 = tile_lo(cx, ex, vp.gw), b1 = tile_hi(cx, ex, vp.gw);
-                const uint32_t b2 = tile_lo(cy, ey, vp.gh), b3 = tile_hi(cy, ey, vp.gh);
-                const uint32_t nt = tiles_touched(vp, b0, b1, b2, b3);
-                if (nt > kMaxTouched) {
-                    // tables that do not fit the view: marked in the word that is
-                    // live anyway, reported and cleared at the end; the Gaussian
-                    // reserves nothing
-                    touched[v] = kTouchedRefused;
-                    continue;
-                }
-                if (nt == 0u) continue;
-                touched[v] = nt;
-                visible[v] = true;
-                bnd[v] = make_ushort4((unsigned short)b0, (unsigned short)b1, (unsigned short)b2, (unsigned short)b3);
-                R1[v] = V1;
-                R2[v] = V2;
-                R4[v] = V4;
-            }
+            R11 = fmaf(-2.0f, yy + zz, 1.0f); R12 = 2.0f * (xy - rz); R13 = 2.0f * (xz + ry);
+            R21 = 2.0f * (xy + rz); R22 = fmaf(-2.0f, xx + zz, 1.0f); R23 = 2.0f * (yz - rx);
+            R31 = 2.0f * (xz - ry); R32 = 2.0f * (yz + rx); R33 = fmaf(-2.0f, xx + yy, 1.0f);
+            rho = 2.0f * fh_logf(opacity * kMinAlphaRcp);
         }
     }
+    // one view's transform, extent, tile bounds and tiles-touched
+    // (kernel_utils.cuh:76-213)
+    auto transform = [&](int v) {
+        if (!zok[v]) return;
+        const ViewParams& vp = a.vp[v];
+        const float sm = vp.scale_mod;
+        const v3 u{(R11 * sc.x) * sm, (R21 * sc.x) * sm, (R31 * sc.x) * sm};
+        const v3 vv{(R12 * sc.y) * sm, (R22 * sc.y) * sm, (R32 * sc.y) * sm};
+        const v3 w{(R13 * sc.z) * sm, (R23 * sc.z) * sm, (R33 * sc.z) * sm};
+        const v4 V4 = frame_row(load_row(vp.VPM, 3), u, vv, w, mu);
+        const v4 d{rho, rho, rho, -1.0f};
+        const float s = dot4(d, mul4(V4, V4));
+        if (s == 0.0f) return;
+        const float rs = 1.0f / s;
+        const v4 f{rs * d.x, rs * d.y, rs * d.z, rs * d.w};
+        const v4 V3 = frame_row(load_row(vp.VPM, 2), u, vv, w, mu);
+        const float cz = dot4(f, mul4(V3, V4));
+        const float ez = sqrtf(fmaxf(fmaf(cz, cz, -dot4(f, mul4(V3, V3))), 0.0f));
+        if (cz - ez < -1.0f || cz + ez > 1.0f) return;
+        const v4 V1 = frame_row(load_row(vp.VPM, 0), u, vv, w, mu);
+        const float cx = dot4(f, mul4(V1, V4));
+        const float ex = sqrtf(fmaxf(fmaf(cx, cx, -dot4(f, mul4(V1, V1))), 0.0f));
+        const v4 V2 = frame_row(load_row(vp.VPM, 1), u, vv, w, mu);
+        const float cy = dot4(f, mul4(V2, V4));
+        const float ey = sqrtf(fmaxf(fmaf(cy, cy, -dot4(f, mul4(V2, V2))), 0.0f));
+        const uint32_t b0 = tile_lo(cx, ex, vp.gw), b1 = tile_hi(cx, ex, vp.gw);
+        const uint32_t b2 = tile_lo(cy, ey, vp.gh), b3 = tile_hi(cy, ey, vp.gh);
+        const uint32_t nt = tiles_touched(vp, b0, b1, b2, b3);
+        if (nt > kMaxTouched) {
+            // tables that do not fit the view: marked in the word that is
+            // live anyway, reported and cleared at the end; the Gaussian
+            // reserves nothing
+            touched[v] = kTouchedRefused;
+            return;
+        }
+        if (nt == 0u) return;
+        touched[v] = nt;
+        visible[v] = true;
+        bnd[v] = make_ushort4((unsigned short)b0, (unsigned short)b1, (unsigned short)b2, (unsigned short)b3);
+        R1[v] = V1;
+        R2[v] = V2;
+        R4[v] = V4;
+    };
 
     // SH colour of the visible Gaussians (sh_utils.cuh:32-69). The wave
-    // gathers the coefficients of up to kShChunk visible Gaussians at a time,
-    // each with one contiguous load across the lanes (all in flight), stages
-    // them in LDS, and the owning lanes evaluate -- instead of 45 scattered
-    // loads per lane.
-    bool any_vis = false;
-#pragma unroll
-    for (int v = 0; v < NV; ++v) any_vis |= visible[v];
+    // copies the coefficients of each visible Gaussian into an LDS slot of
+    // its own with one direct-to-LDS load across the lanes (global_load_lds:
+    // no registers held, all in flight): those of the Gaussians visible in
+    // the first view right after that view's transform, so that they arrive
+    // while the second view's transform runs, the rest after it. A wave with
+    // more visible Gaussians than slots copies the remainder in a further
+    // round. The owning lanes then evaluate from LDS.
     const int active = a.vp[0].active_sh;
     const int total = a.vp[0].total_sh;
     const int shw = min(3 * total, 45);
-    float* wsh = s_sh + (size_t)wid * kShChunk * shw;
-    uint64_t need = __ballot(any_vis && active > 1);
-    const uint64_t lt = lanemask_lt();
-    while (need) {
-        uint64_t chunk = 0;
-        int src[kShChunk];
-#pragma unroll
-        for (int q = 0; q < kShChunk; ++q) {
-            src[q] = need ? __ffsll((unsigned long long)need) - 1 : -1;
-            if (need) {
-                chunk |= 1ull << src[q];
-                need &= need - 1ull;
-            }
+    const bool gather = active > 1;  // degree 0 needs no coefficients
+    float* wsh = s_sh + (size_t)wid * kShSlots * shw;
+    uint32_t used = 0;  // slots taken (wave-uniform)
+    int my_slot = -1;
+    // lanes of m take the next free slots in lane order; returns those left over
+    auto issue = [&](uint64_t m) -> uint64_t {
+        while (m != 0ull && used < (uint32_t)kShSlots) {
+            const int src = __ffsll((unsigned long long)m) - 1;
+            m &= m - 1ull;
+            const uint32_t ip = __builtin_amdgcn_readlane(i, src);
+            if ((int)lane == src) my_slot = (int)used;
+            if ((int)lane < shw)
+                __builtin_amdgcn_global_load_lds(a.shr + (size_t)ip * (3 * total) + lane,
+                                                 (__attribute__((address_space(3))) void*)(wsh + used * shw), 4, 0, 0);
+            ++used;
         }
-        float c[kShChunk];
-#pragma unroll
-        for (int q = 0; q < kShChunk; ++q) {
-            c[q] = 0.0f;
-            if (src[q] >= 0) {
-                const uint32_t ip = __shfl(i, src[q], 64);
-                if ((int)lane < shw) c[q] = a.shr[(size_t)ip * (3 * total) + lane];
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < kShChunk; ++q)
-            if (src[q] >= 0 && (int)lane < shw) wsh[q * shw + lane] = c[q];
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
+        return m;
+    };
+    uint64_t rest = 0ull;
+    transform(0);
+    if (gather) rest = issue(__ballot(visible[0]));
+    if (NV == 2) {
+        transform(NV - 1);
+        if (gather) rest |= issue(__ballot(visible[NV - 1] && !visible[0]));
+    }
+    // evaluates the lanes holding a slot, then frees the slots
+    auto evaluate = [&]() {
+        __builtin_amdgcn_s_waitcnt(0);  // the direct-to-LDS loads have landed
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        if ((chunk >> lane) & 1ull) {
-            const float* coef = wsh + (uint32_t)__popcll(chunk & lt) * shw;
+        if (my_slot >= 0) {
+            const float* coef = wsh + my_slot * shw;
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 if (!visible[v]) continue;
                 const ViewParams& vp = a.vp[v];
                 const v3 cam{vp.cam[0], vp.cam[1], vp.cam[2]};
-                const v3 rgb = sh_color(a.sh0 + 3 * (size_t)i, coef, mu, cam, vp.active_sh, vp.total_sh);
+                const v3 rgb = sh_color(s0, coef, mu, cam, vp.active_sh, vp.total_sh);
                 col[v] = make_float4(rgb.x, rgb.y, rgb.z, opacity);
             }
         }
+        // every LDS read of the slots has returned before a further round's
+        // direct-to-LDS loads may overwrite them
+        __builtin_amdgcn_s_waitcnt(0);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
-    // degree 0 needs no gather
-    if (any_vis && active <= 1) {
+        my_slot = -1;
+        used = 0;
+    };
+    if (gather) {
+        evaluate();
+        while (rest != 0ull) {  // more visible Gaussians than slots (rare)
+            rest = issue(rest);
+            evaluate();
+        }
+    } else {
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             if (!visible[v]) continue;
             const ViewParams& vp = a.vp[v];
             const v3 cam{vp.cam[0], vp.cam[1], vp.cam[2]};
-            const v3 rgb = sh_color(a.sh0 + 3 * (size_t)i, a.shr, mu, cam, vp.active_sh, vp.total_sh);
+            const v3 rgb = sh_color(s0, a.shr, mu, cam, vp.active_sh, vp.total_sh);
             col[v] = make_float4(rgb.x, rgb.y, rgb.z, opacity);
         }
     }
@@ -612,7 +631,7 @@ hipError_t launch_preprocess(const ViewParams* const* vps, const ViewBuffers* vb
         if (n_views == 1) preprocess_aaa_kernel<1><<<blocks, kPrimBlock, 0, s>>>(a);
         else preprocess_aaa_kernel<2><<<blocks, kPrimBlock, 0, s>>>(a);
     } else {
-        const size_t lds = sizeof(float) * (size_t)(kPrimBlock / 64) * kShChunk * (size_t)min(3 * vps[0]->total_sh, 45);
+        const size_t lds = sizeof(float) * (size_t)(kPrimBlock / 64) * kShSlots * (size_t)min(3 * vps[0]->total_sh, 45);
         if (n_views == 1) preprocess_kernel<1><<<blocks, kPrimBlock, lds, s>>>(a);
         else preprocess_kernel<2><<<blocks, kPrimBlock, lds, s>>>(a);
     }
