@@ -166,5 +166,12 @@ FH_HD uint32_t fh_mask_row_bits(const FhMaskPrep& p, float row_dy, float w, int 
     const float b = fh_med3(ceilf(uc - w), 0.0f, fc);
     const float e = fh_med3(floorf(uc + w) + 1.0f, 0.0f, fc);
     const uint32_t ib = (uint32_t)b, n = (uint32_t)fh_fmax(e - b, 0.0f);
+#if defined(__HIP_DEVICE_COMPILE__)
+    // n, ib <= cols <= 8: v_bfm_b32 gives ((1 << n) - 1) << ib in one instruction
+    uint32_t m;
+    __asm__("v_bfm_b32 %0, %1, %2" : "=v"(m) : "v"(n), "v"(ib));
+    return m;
+#else
     return ((1u << n) - 1u) << ib;
+#endif
 }

@@ -274,7 +274,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 5 :
         // Pass 1: which staged Gaussians can reach this pixel: this lane's bit
         // of each staged Gaussian's sample mask, a superset of the pairs the
         // reference's cutoff and alpha tests keep. The answers are shifted in,
-        // first Gaussian highest: bit 63 - jj of `reach` is staged Gaussian jj.
+        // first Gaussian highest: staged Gaussians 0-31 in ra, 32-63 in rb.
         auto may_reach = [&](uint32_t e) -> uint32_t { return (sM[2u * e + m_word] >> m_shift) & 1u; };
         const int na = nmax < 32 ? nmax : 32;
         uint32_t ra = 0u, rb = 0u;
@@ -282,17 +282,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 5 :
         for (int jj = 0; jj < na; ++jj) ra = ra + ra + may_reach(gbase + (uint32_t)jj);
 #pragma unroll 4
         for (int jj = 32; jj < nmax; ++jj) rb = rb + rb + may_reach(gbase + (uint32_t)jj);
-        uint64_t reach = (uint64_t)(na > 0 ? ra << (32 - na) : 0u) << 32;
-        if (nmax > 32) reach |= rb << (64 - nmax);
         // Pass 2: the reference's evaluation (inference.cuh:200-238) of those
         // Gaussians, each lane walking its own bits in ascending order, so
         // every pixel sees the same Gaussians in the same order as a walk
         // over the whole batch, and the wave runs as many steps as its
         // busiest lane needs instead of one per staged Gaussian.
-        while (reach != 0ull) {
-            const uint32_t jj = (uint32_t)__builtin_clzll(reach);
-            reach ^= (1ull << 63) >> jj;
-            const uint32_t e = gbase + jj;
+        auto evaluate = [&](uint32_t e) {
             const float4 W1 = sV1[e], W2 = sV2[e], W4 = sV4[e];
             const float Px0 = fmaf(-W4.x, pxf, W1.x), Px1 = fmaf(-W4.y, pxf, W1.y);
             const float Px2 = fmaf(-W4.z, pxf, W1.z), Px3 = fmaf(-W4.w, pxf, W1.w);
@@ -303,13 +298,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 5 :
             const v3 d = cross3(v3{Px0, Px1, Px2}, v3{Py0, Py1, Py2});
             const float num = dot3(m, m);
             const float den = dot3(d, d);
-            if (num > kMaxCutoffSq * den) continue;
+            if (num > kMaxCutoffSq * den) return;
             // num <= 11.08 den here, so the exponent lies in [-6, 0] (or is NaN)
             const float rcp = rcp_rn(den);
             const float G = fh_expf_m64_gfx((-0.5f * num) * rcp);
             const float4 Cj = sC[e];
             const float alpha = fminf(Cj.w * G, kMaxFragmentAlpha);
-            if (alpha < kMinAlpha) continue;
+            if (alpha < kMinAlpha) return;
             float p0 = Cj.x * alpha, p1 = Cj.y * alpha, p2 = Cj.z * alpha, p3 = alpha;
             // the core test of inference.cuh:225 with its alpha half first: the
             // depth only matters for fragments that may enter the core
@@ -341,6 +336,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 5 :
             const float tt = 1.0f - p3;
             t0 = t0 + p0; t1 = t1 + p1; t2 = t2 + p2; t3 = t3 + p3;
             T_tail = T_tail * tt;
+        };
+        if (nmax <= 32) {
+            // one word (the usual case): bit 31 - jj is staged Gaussian jj
+            uint32_t reach = na > 0 ? ra << (32 - na) : 0u;
+            while (reach != 0u) {
+                const uint32_t jj = (uint32_t)__builtin_clz(reach);
+                reach ^= 0x80000000u >> jj;
+                evaluate(gbase + jj);
+            }
+        } else {
+            uint64_t reach = ((uint64_t)ra << 32) | (rb << (64 - nmax));
+            while (reach != 0ull) {
+                const uint32_t jj = (uint32_t)__builtin_clzll(reach);
+                reach ^= (1ull << 63) >> jj;
+                evaluate(gbase + jj);
+            }
         }
     }
     if (!inside) return;
