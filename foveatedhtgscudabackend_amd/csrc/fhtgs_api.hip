@@ -434,8 +434,10 @@ int join_branches(fhtgs_context* h, int nb, hipStream_t s) {
 // preprocess, scan. pbytes: the parameter block's size. Untimed, with
 // branches, the upload and the tile classifications (one workgroup each) run
 // on the side streams beside the preprocess (which takes its parameters as
-// kernel arguments), and the scans side by side after it; they are joined at
-// the end, before anything that reads the uploaded block. sort_clear: the
+// kernel arguments) and are joined into the stream behind it -- they finish
+// long before it does -- so the views' scans (one launch, one workgroup a
+// view) follow the preprocess directly and phase 2 starts right after them,
+// with no round trip through a side stream. sort_clear: the
 // instance buffers are already sized (device counts, graph), so their sort
 // counters are cleared here, after each view's tile classification, instead
 // of in phase 2.
@@ -487,6 +489,7 @@ int enqueue_phase1(fhtgs_context* h, const std::vector<PreGroup>& groups, const 
             for (int b = 2; b < nb; ++b) HCHK(hipStreamWaitEvent(h->side[b - 1], h->ev_params, 0));
         }
         if ((rc = tiles())) return rc;
+        if ((rc = join_branches(h, nb, s))) return rc;
     } else {
         HCHK(hipMemcpyAsync(h->params.ptr, h->params_host, pbytes, hipMemcpyHostToDevice, s));
         if ((rc = tiles())) return rc;
@@ -494,15 +497,18 @@ int enqueue_phase1(fhtgs_context* h, const std::vector<PreGroup>& groups, const 
         if ((rc = preprocess())) return rc;
     }
     if (timed && (rc = record(h, 3, s))) return rc;
-    // the views' block-sum scans (one workgroup each) side by side
-    if ((rc = fork_branches(h, nb, s))) return rc;
+    // the views' block-sum scans
+    std::vector<const ViewBuffers*> svb;
+    std::vector<uint32_t> snb;
     for (int v = 0; v < n_views; ++v) {
         const int64_t N = (*h->cur[v]).n_prims;
-        if (N > 0)
-            HCHK(launch_scan_blocks((*h->cur[v]).vb, (uint32_t)((N + kPrimBlock - 1) / kPrimBlock),
-                                    branch_stream(h, v % nb, s)));
+        if (N > 0) {
+            svb.push_back(&(*h->cur[v]).vb);
+            snb.push_back((uint32_t)((N + kPrimBlock - 1) / kPrimBlock));
+        }
     }
-    return join_branches(h, nb, s);
+    if (!svb.empty()) HCHK(launch_scan_blocks(svb.data(), snb.data(), (int)svb.size(), s));
+    return FHTGS_OK;
 }
 
 // Phase 2: instances, sort, ranges, output clear + blend, upsample.

@@ -152,7 +152,9 @@ hipError_t launch_tiles(const ViewParams& vp, const ViewParams* dvp, const ViewB
 // The views' parameters and the scene go by value into the kernel arguments.
 hipError_t launch_preprocess(const ViewParams* const* vps, const ViewBuffers* vbs, int n_views,
                              const SceneArgs& scene, uint32_t n, hipStream_t s);
-hipError_t launch_scan_blocks(const ViewBuffers& vb, uint32_t n_blocks, hipStream_t s);
+// The block sums of n_views views, one workgroup each, in one launch.
+hipError_t launch_scan_blocks(const ViewBuffers* const* vbs, const uint32_t* n_blocks, int n_views,
+                              hipStream_t s);
 hipError_t launch_sort_clear(const InstanceBuffers& ib, const SortPlan& plan, int64_t tiles,
                              hipStream_t s);
 hipError_t launch_instances(const ViewParams* dvp, const ViewBuffers& vb, const InstanceBuffers& ib,

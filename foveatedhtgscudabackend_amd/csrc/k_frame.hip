@@ -184,13 +184,22 @@ hipError_t launch_tiles(const ViewParams& vp, const ViewParams* dvp, const ViewB
 // the instance count I (inference.cu:226-236). Chunks of 8192 sums are
 // loaded coalesced into LDS (all eight loads of a thread in flight), each
 // thread scans eight consecutive sums there, and the chunk is stored back
-// coalesced; a running total carries from chunk to chunk.
+// coalesced; a running total carries from chunk to chunk. One workgroup per
+// view: the views of a frame are scanned by one launch.
 constexpr uint32_t kScanPer = 8;
 constexpr uint32_t kScanThreads = 1024;
 constexpr uint32_t kScanChunk = kScanPer * kScanThreads;
+constexpr int kScanViews = 8;
+struct ScanArgs {
+    uint32_t* sums[kScanViews];
+    uint32_t* frame_info[kScanViews];
+    uint32_t nb[kScanViews];
+};
 
-__global__ __launch_bounds__(kScanThreads) void scan_blocks_kernel(uint32_t* sums, uint32_t nb,
-                                                                   uint32_t* frame_info) {
+__global__ __launch_bounds__(kScanThreads) void scan_blocks_kernel(ScanArgs a) {
+    uint32_t* const sums = a.sums[blockIdx.x];
+    uint32_t* const frame_info = a.frame_info[blockIdx.x];
+    const uint32_t nb = a.nb[blockIdx.x];
     __shared__ uint32_t s_v[kScanChunk];
     __shared__ uint32_t s_w[kScanThreads / 64];
     __shared__ unsigned long long s_w64[kScanThreads / 64];
@@ -250,9 +259,22 @@ __global__ __launch_bounds__(kScanThreads) void scan_blocks_kernel(uint32_t* sum
     }
 }
 
-hipError_t launch_scan_blocks(const ViewBuffers& vb, uint32_t n_blocks, hipStream_t s) {
-    scan_blocks_kernel<<<1, kScanThreads, 0, s>>>(vb.block_sums, n_blocks, vb.frame_info);
-    return hipGetLastError();
+hipError_t launch_scan_blocks(const ViewBuffers* const* vbs, const uint32_t* n_blocks, int n_views,
+                              hipStream_t s) {
+    for (int v0 = 0; v0 < n_views; v0 += kScanViews) {
+        const int nv = std::min(kScanViews, n_views - v0);
+        ScanArgs a;
+        for (int k = 0; k < kScanViews; ++k) {
+            const int v = v0 + (k < nv ? k : 0);
+            a.sums[k] = vbs[v]->block_sums;
+            a.frame_info[k] = vbs[v]->frame_info;
+            a.nb[k] = n_blocks[v];
+        }
+        scan_blocks_kernel<<<nv, kScanThreads, 0, s>>>(a);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 // Render-path output clear (inference_api.cu:79-83 allocates zeros): only the
