@@ -129,8 +129,9 @@ def test_workspace_query_sizes_a_slot_from_counts_on_the_host():
     """fhtgs_workspace_query gives a slot's bytes from the view's geometry and
     the element counts, before anything exists on the device (pointers may be
     NULL): the per-Gaussian part grows by the buffers the preprocess writes
-    (tiles touched 4, bounds 8, record row 64, ellipse record 20, block sums
-    4/256 B), the instance part by double-buffered 32-bit keys and values."""
+    (tiles touched 4, bounds 8, record row 64, ellipse record as one 32-B
+    entry, block sums 4/256 B), the instance part by double-buffered 32-bit
+    keys and values."""
     from foveatedhtgscudabackend_amd import _lib
     geo = dict(K=8, width=2064, height=2208, foveation_radius_px=232, blend_width_px=77, quarter_radius_px=606)
     v = _lib.View(**geo)
@@ -138,7 +139,7 @@ def test_workspace_query_sizes_a_slot_from_counts_on_the_host():
     f1, i1 = _lib.workspace_query(v, 3_000_000, 4_800_000)
     assert 0 < f0 < f1 and 0 < i0 < i1
     per_gauss = (f1 - f0) / 3_000_000
-    assert 4 + 8 + 64 + 20 <= per_gauss <= 4 + 8 + 64 + 20 + 1, per_gauss
+    assert 4 + 8 + 64 + 32 <= per_gauss <= 4 + 8 + 64 + 32 + 1, per_gauss
     per_inst = (i1 - i0) / 4_800_000
     assert 16 <= per_inst <= 17, per_inst  # 2 x (4-B key + 4-B value), plus look-back status
     # the tile part follows the image size; tables brought by the caller
