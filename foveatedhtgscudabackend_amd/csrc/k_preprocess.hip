@@ -18,8 +18,10 @@ namespace fhtgs {
 
 constexpr int kMaxViewsPerPass = 2;
 // LDS slots of SH coefficients per wave (a visible Gaussian each; a wave
-// with more visible Gaussians copies the rest in a further round)
-constexpr int kShSlots = 36;
+// with more visible Gaussians copies the rest in a further round). 44 slots
+// of 45 floats are 31.7 KB per workgroup of four waves: the most that still
+// lets the five workgroups per CU which the registers allow share the LDS.
+constexpr int kShSlots = 44;
 
 // Kernel argument: the views' parameters and the scene's tensors by value
 // (pointers read from kernel arguments are global ones to the compiler). The
