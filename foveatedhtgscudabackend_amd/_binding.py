@@ -33,6 +33,16 @@ def _dev_ptr(t: torch.Tensor, name: str, dtypes, device) -> tuple[int, torch.Ten
 
 
 _F32 = (torch.float32,)
+
+# the current stream's handle: torch's raw accessor where it exists (no
+# Stream object per call), else the public one
+_raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+
+
+def _current_stream(index: int) -> int:
+    if _raw_stream is not None:
+        return _raw_stream(index)
+    return torch.cuda.current_stream(index).cuda_stream
 _U32 = tuple(d for d in (getattr(torch, "uint32", None), torch.int32) if d is not None)
 
 
@@ -81,7 +91,10 @@ def view_struct(device, M, VPM, VPR_inv, cam_position, gaze_position, visibility
     # the reference reads the gaze through a host pointer (inference.cu:79-82)
     if gaze_position.numel() != 2:
         raise ValueError("gaze_position must have 2 elements")
-    kw["gaze_x"], kw["gaze_y"] = gaze_position.detach().to("cpu", torch.float32).reshape(-1).tolist()
+    g = gaze_position
+    if g.is_cuda or g.dtype is not torch.float32:
+        g = g.detach().to("cpu", torch.float32)
+    kw["gaze_x"], kw["gaze_y"] = g.tolist() if g.dim() == 1 else g.reshape(-1).tolist()
     gw, gh = (width + 15) // 16, (height + 15) // 16
     # the library's own up-front checks (fhtgs_validate_view), before any allocation
     if gw * gh > _lib.MAX_LARGE_TILES:
@@ -166,8 +179,9 @@ def render_views(scene_args: tuple, view_args: list, share_preprocess: bool, cle
                              blend_width_px, clear_output, keep=keep, quarter_radius_px=quarter_radius_px)
         views.append(v)
         images.append(img)
-    handle = _lib.handle_for(device.index if device.index is not None else torch.cuda.current_device())
-    stream = torch.cuda.current_stream(device).cuda_stream
+    index = device.index if device.index is not None else torch.cuda.current_device()
+    handle = _lib.handle_for(index)
+    stream = _current_stream(index)
     handle.render(scene, views, share_preprocess, stream)
     del keep
     return images

@@ -58,8 +58,9 @@ def _marshal(scenes: list, views: list):
     if any(given) and not all(given):
         raise ValueError("give a slot for every view of the batch or for none")
     slots = [bv.slot for bv in views] if all(given) else None
-    handle = _lib.handle_for(device.index if device.index is not None else torch.cuda.current_device())
-    stream = torch.cuda.current_stream(device).cuda_stream
+    dev_index = device.index if device.index is not None else torch.cuda.current_device()
+    handle = _lib.handle_for(dev_index)
+    stream = _binding._current_stream(dev_index)
     return handle, structs, vstructs, index, slots, images, stream, keep
 
 

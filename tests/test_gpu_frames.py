@@ -383,3 +383,31 @@ def test_tables_of_another_radius_fail_with_an_error_not_a_fault():
         h.render_batch([scene], [again], [0], stream.cuda_stream, [0])
         torch.cuda.synchronize()
         _same(img.cpu().numpy(), ref, f"frame after the rejected one, mode {mode}")
+
+
+def test_render_follows_the_callers_current_stream(restore_mode):
+    """The binding passes the current stream (torch's raw accessor, the same
+    handle as torch.cuda.current_stream().cuda_stream): a frame rendered
+    inside `with torch.cuda.stream(side)` runs on that stream, in graph mode
+    too, and equals the frame rendered on the default stream; a gaze given as
+    a (1, 2) tensor reads the same as a (2,) one."""
+    from foveatedhtgscudabackend_amd import FoveatedHTGSRasterizer, _binding
+    from foveatedhtgscudabackend_amd import scene as S
+    lib, h = restore_mode
+    assert _binding._current_stream(0) == torch.cuda.current_stream(0).cuda_stream
+    case = Case("stream", 30000, 640, 480, seed=71)
+    sc = S.make_scene(case.n, seed=case.seed, cube=case.cube)
+    args = S.scene_to_torch(sc, "cuda")
+    cam = S.orbit_camera(case.width, case.height, case.fov_deg, seed=1071)
+    st = S.settings_for(cam, S.seeded_gaze(cam, 2071))
+    ref = _render(args, st)
+    side = torch.cuda.Stream()
+    for mode in (lib.FRAME_EXACT, lib.FRAME_GRAPH):
+        h.set_option(lib.OPT_FRAME_MODE, mode)
+        with torch.cuda.stream(side):
+            assert _binding._current_stream(0) == side.cuda_stream
+            for _ in range(2):
+                img = FoveatedHTGSRasterizer(375, 125, 0).render(*args, st, False, False, False)
+        side.synchronize()
+        _same(img.cpu().numpy(), ref, f"side stream, mode {mode}")
+    _same(_render(args, st._replace(gaze_position=st.gaze_position.reshape(1, 2))), ref, "gaze as a (1, 2) tensor")
