@@ -273,22 +273,27 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 5 :
         if (!inside) continue;
         // Pass 1: which staged Gaussians can reach this pixel: this lane's bit
         // of each staged Gaussian's sample mask, a superset of the pairs the
-        // reference's cutoff and alpha tests keep. The answers are shifted in,
-        // first Gaussian highest: staged Gaussians 0-31 in ra, 32-63 in rb.
-        auto may_reach = [&](uint32_t e) -> uint32_t { return (sM[2u * e + m_word] >> m_shift) & 1u; };
+        // reference's cutoff and alpha tests keep. Bit jj of ra is staged
+        // Gaussian jj, bit jj - 32 of rb staged Gaussian jj (jj >= 32).
+        auto may_reach = [&](uint32_t e) -> uint32_t { return __builtin_amdgcn_ubfe(sM[2u * e + m_word], m_shift, 1u); };
         const int na = nmax < 32 ? nmax : 32;
         uint32_t ra = 0u, rb = 0u;
 #pragma unroll 4
-        for (int jj = 0; jj < na; ++jj) ra = ra + ra + may_reach(gbase + (uint32_t)jj);
+        for (int jj = 0; jj < na; ++jj) ra |= may_reach(gbase + (uint32_t)jj) << jj;
 #pragma unroll 4
-        for (int jj = 32; jj < nmax; ++jj) rb = rb + rb + may_reach(gbase + (uint32_t)jj);
+        for (int jj = 32; jj < nmax; ++jj) rb |= may_reach(gbase + (uint32_t)jj) << (jj - 32);
         // Pass 2: the reference's evaluation (inference.cuh:200-238) of those
         // Gaussians, each lane walking its own bits in ascending order, so
         // every pixel sees the same Gaussians in the same order as a walk
         // over the whole batch, and the wave runs as many steps as its
         // busiest lane needs instead of one per staged Gaussian.
-        auto evaluate = [&](uint32_t e) {
-            const float4 W1 = sV1[e], W2 = sV2[e], W4 = sV4[e];
+        // the group's part of the batch, as byte offsets from each array
+        const uint32_t gbyte = gbase * (uint32_t)sizeof(float4);
+        auto slot = [&](const float4* arr, uint32_t jj) -> float4 {
+            return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(arr) + (gbyte + (jj << 4)));
+        };
+        auto evaluate = [&](uint32_t jj) {
+            const float4 W1 = slot(sV1, jj), W2 = slot(sV2, jj), W4 = slot(sV4, jj);
             const float Px0 = fmaf(-W4.x, pxf, W1.x), Px1 = fmaf(-W4.y, pxf, W1.y);
             const float Px2 = fmaf(-W4.z, pxf, W1.z), Px3 = fmaf(-W4.w, pxf, W1.w);
             const float Py0 = fmaf(-W4.x, pyf, W2.x), Py1 = fmaf(-W4.y, pyf, W2.y);
@@ -302,7 +307,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 5 :
             // num <= 11.08 den here, so the exponent lies in [-6, 0] (or is NaN)
             const float rcp = rcp_rn(den);
             const float G = fh_expf_m64_gfx((-0.5f * num) * rcp);
-            const float4 Cj = sC[e];
+            const float4 Cj = slot(sC, jj);
             const float alpha = fminf(Cj.w * G, kMaxFragmentAlpha);
             if (alpha < kMinAlpha) return;
             float p0 = Cj.x * alpha, p1 = Cj.y * alpha, p2 = Cj.z * alpha, p3 = alpha;
@@ -337,20 +342,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 5 :
             t0 = t0 + p0; t1 = t1 + p1; t2 = t2 + p2; t3 = t3 + p3;
             T_tail = T_tail * tt;
         };
+        // lowest set bit first, i.e. in list order
         if (nmax <= 32) {
-            // one word (the usual case): bit 31 - jj is staged Gaussian jj
-            uint32_t reach = na > 0 ? ra << (32 - na) : 0u;
+            // one word (the usual case)
+            uint32_t reach = ra;
             while (reach != 0u) {
-                const uint32_t jj = (uint32_t)__builtin_clz(reach);
-                reach ^= 0x80000000u >> jj;
-                evaluate(gbase + jj);
+                const uint32_t jj = (uint32_t)__builtin_ctz(reach);
+                reach &= reach - 1u;
+                evaluate(jj);
             }
         } else {
-            uint64_t reach = ((uint64_t)ra << 32) | (rb << (64 - nmax));
+            uint64_t reach = (uint64_t)ra | ((uint64_t)rb << 32);
             while (reach != 0ull) {
-                const uint32_t jj = (uint32_t)__builtin_clzll(reach);
-                reach ^= (1ull << 63) >> jj;
-                evaluate(gbase + jj);
+                const uint32_t jj = (uint32_t)__builtin_ctzll(reach);
+                reach &= reach - 1ull;
+                evaluate(jj);
             }
         }
     }
