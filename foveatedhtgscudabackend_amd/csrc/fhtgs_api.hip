@@ -1019,7 +1019,11 @@ int validate_call(fhtgs_context* h, const fhtgs_scene* scenes, int n_scenes, con
         if (rc) return rc;
         if (view_scene[v] < 0 || view_scene[v] >= n_scenes)
             return set_err(h, FHTGS_E_INVALID, "view " + std::to_string(v) + ": scene index out of range");
-        if (views[v].active_sh_bases > 1 && views[v].active_sh_bases > scenes[view_scene[v]].total_sh_bases + 1)
+        // the degree that active_sh_bases selects reads 3, 8 or 15 coefficients
+        // per Gaussian (sh_utils.cuh:32-69); fewer given would be read past
+        const int act = views[v].active_sh_bases;
+        const int need = act <= 1 ? 0 : (act <= 4 ? 3 : (act <= 9 ? 8 : 15));
+        if (need > scenes[view_scene[v]].total_sh_bases)
             return set_err(h, FHTGS_E_INVALID, "active_sh_bases exceeds the SH coefficients given");
         if (view_slot) {
             const int sl = view_slot[v];

@@ -114,7 +114,8 @@ typedef struct {
     int32_t background_width;  /* texture w */
     int32_t background_height; /* texture h */
     int32_t K;
-    int32_t active_sh_bases;
+    int32_t active_sh_bases; /* 1 / <= 4 / <= 9 / more: SH degree 0..3, which needs
+                                total_sh_bases >= 0 / 3 / 8 / 15 (else FHTGS_E_INVALID) */
     int32_t width;
     int32_t height;
     float focal_x, focal_y, center_x, center_y;

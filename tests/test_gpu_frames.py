@@ -411,3 +411,30 @@ def test_render_follows_the_callers_current_stream(restore_mode):
         side.synchronize()
         _same(img.cpu().numpy(), ref, f"side stream, mode {mode}")
     _same(_render(args, st._replace(gaze_position=st.gaze_position.reshape(1, 2))), ref, "gaze as a (1, 2) tensor")
+
+
+def test_sh_degree_needs_its_coefficients():
+    """The SH degree that active_sh_bases selects reads 3, 8 or 15 coefficients
+    per Gaussian: a scene with fewer is refused before any device work (degree
+    3 over 8 bases, degree 2 over 3), and the degree the bases cover renders
+    as the same view with all 15 bases cut to it."""
+    from foveatedhtgscudabackend_amd import FoveatedHTGSRasterizer
+    from foveatedhtgscudabackend_amd import scene as S
+    case = Case("sh_bases", 20000, 320, 240, seed=81)
+    sc = S.make_scene(case.n, seed=case.seed, cube=case.cube)
+    args = list(S.scene_to_torch(sc, "cuda"))
+    cam = S.orbit_camera(case.width, case.height, case.fov_deg, seed=1081)
+    gaze = S.seeded_gaze(cam, 2081)
+    r = FoveatedHTGSRasterizer(375, 125, 0)
+    full = args[5]
+    for bases, active in ((8, 10), (8, 16), (3, 5)):
+        args[5] = full[:, :bases, :].contiguous()
+        with pytest.raises(ValueError, match="active_sh_bases exceeds"):
+            r.render(*args, S.settings_for(cam, gaze, active_sh_bases=active), False, False, False)
+    # degree 2 (9 bases active) reads bases 0..7 only: 8 given equals 15 given
+    args[5] = full[:, :8, :].contiguous()
+    got = r.render(*args, S.settings_for(cam, gaze, active_sh_bases=9), False, False, False)
+    args[5] = full
+    want = r.render(*args, S.settings_for(cam, gaze, active_sh_bases=9), False, False, False)
+    torch.cuda.synchronize()
+    _same(got.cpu().numpy(), want.cpu().numpy(), "degree 2 over 8 or 15 bases")
