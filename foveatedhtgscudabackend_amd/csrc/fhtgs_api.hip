@@ -243,6 +243,7 @@ void carve_fixed(void* base, int64_t N, int n_large, int n_small, ViewBuffers& v
     vb.prim = c.take<float4>((size_t)N * 4);
     vb.ell = c.take<float4>((size_t)N * 2);
     vb.block_sums = c.take<uint32_t>(std::max<size_t>(nb, 1));
+    vb.block_vis = c.take<uint32_t>(std::max<size_t>(nb, 1));
     vb.tab_mask = vb.tab_vis_sat = vb.tab_fovea_sat = nullptr;
     if (R >= 0) {
         vb.tab_mask = c.take<uint32_t>((size_t)(n_large + 31) / 32);
@@ -432,9 +433,10 @@ int join_branches(fhtgs_context* h, int nb, hipStream_t s) {
 
 // Phase 1 of a frame: parameter upload, clears, tile classification,
 // preprocess, scan. pbytes: the parameter block's size. Untimed, with
-// branches, the upload and the tile classifications (one workgroup each) run
-// on the side streams beside the preprocess (which takes its parameters as
-// kernel arguments) and are joined into the stream behind it -- they finish
+// branches, the preprocess (which takes its parameters as kernel arguments
+// and writes its counts per block) is the frame's first kernel, and the
+// clears, the upload and the tile classifications (one workgroup each) run
+// on the side streams beside it and are joined into the stream behind it -- they finish
 // long before it does -- so the views' scans (one launch, one workgroup a
 // view) follow the preprocess directly and phase 2 starts right after them,
 // with no round trip through a side stream. sort_clear: the
@@ -456,7 +458,7 @@ int enqueue_phase1(fhtgs_context* h, const std::vector<PreGroup>& groups, const 
         cvp[(size_t)v] = &sl.vp;
         cvb[(size_t)v] = &sl.vb;
     }
-    HCHK(launch_clear_frames(cvp.data(), cvb.data(), n_views, s));
+    if (nb == 1) HCHK(launch_clear_frames(cvp.data(), cvb.data(), n_views, s));
     if (timed && (rc = record(h, 1, s))) return rc;
     auto tiles = [&]() -> int {
         for (int v = 0; v < n_views; ++v) {
@@ -478,11 +480,14 @@ int enqueue_phase1(fhtgs_context* h, const std::vector<PreGroup>& groups, const 
         return FHTGS_OK;
     };
     if (nb > 1) {
-        // the preprocess is enqueued first; the upload and then the tile
-        // kernels follow on the side streams, after the clears
+        // the preprocess is enqueued first; the clears, the upload and then
+        // the tile kernels follow on the side streams
         HCHK(hipEventRecord(h->ev_fork, s));
         if ((rc = preprocess())) return rc;
         HCHK(hipStreamWaitEvent(h->side[0], h->ev_fork, 0));
+        // the preprocess reads no frame-info word (it writes per-block counts),
+        // so the clears run here, beside it, ahead of the tile kernels
+        HCHK(launch_clear_frames(cvp.data(), cvb.data(), n_views, h->side[0]));
         HCHK(hipMemcpyAsync(h->params.ptr, h->params_host, pbytes, hipMemcpyHostToDevice, h->side[0]));
         if (nb > 2) {
             HCHK(hipEventRecord(h->ev_params, h->side[0]));

@@ -59,9 +59,12 @@ enum FrameInfo : int {
     FI_ERROR = 11,        // bit 0: tiles-touched under-count, bit 1: sort inconsistency,
                           // bit 2: instance capacity exceeded, bit 3: culling tables that
                           // do not fit the view (impossible tiles-touched)
-    FI_N_VISIBLE = 12,    // preprocess survivors (atomic)
+    FI_N_VISIBLE = 12,    // preprocess survivors (summed by the block scan)
     FI_WORDS = 16
 };
+// ViewBuffers::block_vis, bit 31: a Gaussian of the block had a tiles-touched
+// the culling tables cannot give (FI_ERROR bit 3 once the block scan has run)
+constexpr uint32_t kVisRefused = 0x80000000u;
 
 struct ViewParams {
     const float* M;
@@ -113,6 +116,7 @@ struct ViewBuffers {
     float4* ell;              // 2N: support-ellipse record (fh_ellipse_record), 32 B per Gaussian:
                               // (cx, cy, m11, m22), (m12, 0, 0, 0) -- one memory sector per gather
     uint32_t* block_sums;     // ceil(N / kPrimBlock), then exclusive offsets
+    uint32_t* block_vis;      // ceil(N / kPrimBlock): visible Gaussians, bit 31 a refused table
     // culling tables built on the device when the view brings none
     uint32_t* tab_mask;       // ceil(n_large / 32)
     uint32_t* tab_vis_sat;    // (gw+1)*(gh+1)

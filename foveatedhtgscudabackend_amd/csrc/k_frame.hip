@@ -185,19 +185,23 @@ hipError_t launch_tiles(const ViewParams& vp, const ViewParams* dvp, const ViewB
 // loaded coalesced into LDS (all eight loads of a thread in flight), each
 // thread scans eight consecutive sums there, and the chunk is stored back
 // coalesced; a running total carries from chunk to chunk. One workgroup per
-// view: the views of a frame are scanned by one launch.
+// view: the views of a frame are scanned by one launch. The same workgroup
+// sums the per-block visible counts into FI_N_VISIBLE and raises FI_ERROR
+// bit 3 for a block that flagged a refused tiles-touched.
 constexpr uint32_t kScanPer = 8;
 constexpr uint32_t kScanThreads = 1024;
 constexpr uint32_t kScanChunk = kScanPer * kScanThreads;
 constexpr int kScanViews = 8;
 struct ScanArgs {
     uint32_t* sums[kScanViews];
+    const uint32_t* vis[kScanViews];
     uint32_t* frame_info[kScanViews];
     uint32_t nb[kScanViews];
 };
 
 __global__ __launch_bounds__(kScanThreads) void scan_blocks_kernel(ScanArgs a) {
     uint32_t* const sums = a.sums[blockIdx.x];
+    const uint32_t* const vis = a.vis[blockIdx.x];
     uint32_t* const frame_info = a.frame_info[blockIdx.x];
     const uint32_t nb = a.nb[blockIdx.x];
     __shared__ uint32_t s_v[kScanChunk];
@@ -206,12 +210,19 @@ __global__ __launch_bounds__(kScanThreads) void scan_blocks_kernel(ScanArgs a) {
     const uint32_t tid = threadIdx.x, lane = lane_id(), wid = tid >> 6;
     uint32_t carry = 0;
     unsigned long long mine = 0;  // this thread's share of the total, without wrap
+    uint32_t n_vis = 0, refused = 0;  // this thread's share of the visible counts, flags
     for (uint32_t c0 = 0; c0 < nb; c0 += kScanChunk) {
-        uint32_t x[kScanPer];
+        uint32_t x[kScanPer], y[kScanPer];
 #pragma unroll
         for (uint32_t k = 0; k < kScanPer; ++k) {
             const uint32_t b = c0 + k * kScanThreads + tid;
             x[k] = b < nb ? sums[b] : 0u;
+            y[k] = b < nb ? vis[b] : 0u;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kScanPer; ++k) {
+            n_vis += y[k] & ~kVisRefused;
+            refused |= y[k] & kVisRefused;
         }
 #pragma unroll
         for (uint32_t k = 0; k < kScanPer; ++k) s_v[k * kScanThreads + tid] = x[k];
@@ -250,12 +261,24 @@ __global__ __launch_bounds__(kScanThreads) void scan_blocks_kernel(ScanArgs a) {
     // (possible only with tiles-touched near its bound everywhere) is
     // published as 0xffffffff, which every frame mode rejects
     for (int o = 32; o > 0; o >>= 1) mine += __shfl_xor(mine, o, 64);
-    if (lane == 0) s_w64[wid] = mine;
+    const uint32_t wvis = wave_sum(n_vis);
+    const bool wref = __ballot(refused != 0u) != 0ull;
+    if (lane == 0) {
+        s_w64[wid] = mine;
+        s_w[wid] = wvis | (wref ? kVisRefused : 0u);
+    }
     __syncthreads();
     if (tid == 0) {
         unsigned long long total = 0;
-        for (uint32_t w = 0; w < kScanThreads / 64; ++w) total += s_w64[w];
+        uint32_t nv = 0, ref = 0;
+        for (uint32_t w = 0; w < kScanThreads / 64; ++w) {
+            total += s_w64[w];
+            nv += s_w[w] & ~kVisRefused;
+            ref |= s_w[w] & kVisRefused;
+        }
         frame_info[FI_INSTANCES] = total > 0xffffffffull ? 0xffffffffu : carry;
+        frame_info[FI_N_VISIBLE] = nv;
+        if (ref) atomicOr(&frame_info[FI_ERROR], 8u);
     }
 }
 
@@ -267,6 +290,7 @@ hipError_t launch_scan_blocks(const ViewBuffers* const* vbs, const uint32_t* n_b
         for (int k = 0; k < kScanViews; ++k) {
             const int v = v0 + (k < nv ? k : 0);
             a.sums[k] = vbs[v]->block_sums;
+            a.vis[k] = vbs[v]->block_vis;
             a.frame_info[k] = vbs[v]->frame_info;
             a.nb[k] = n_blocks[v];
         }

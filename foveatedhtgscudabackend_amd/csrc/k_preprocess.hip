@@ -392,10 +392,10 @@ __global__ __launch_bounds__(kPrimBlock) void preprocess_aaa_kernel(PreprocessAr
     for (int v = 0; v < NV; ++v) {
         const bool refused = touched[v] == kTouchedRefused;
         if (refused) touched[v] = 0u;
-        if (__ballot(refused) != 0ull && lane == 0) atomicOr(&a.vb[v].frame_info[FI_ERROR], 8u);
+        const uint32_t refused_w = __ballot(refused) != 0ull ? kVisRefused : 0u;
         if (valid) a.vb[v].n_touched[i] = touched[v];
         const uint32_t ws = wave_sum(touched[v]);
-        const uint32_t wv = (uint32_t)__popcll(__ballot(visible[v]));
+        const uint32_t wv = (uint32_t)__popcll(__ballot(visible[v])) | refused_w;
         if (lane == 0) { s_sum[v][wid] = ws; s_vis[v][wid] = wv; }
     }
     __syncthreads();
@@ -404,9 +404,16 @@ __global__ __launch_bounds__(kPrimBlock) void preprocess_aaa_kernel(PreprocessAr
     for (int v = 0; v < NV; ++v) {
         if ((int)threadIdx.x != v) continue;
         uint32_t tot = 0, vis = 0;
-        for (int w = 0; w < kPrimBlock / 64; ++w) { tot += s_sum[v][w]; vis += s_vis[v][w]; }
+        uint32_t flag = 0;
+        for (int w = 0; w < kPrimBlock / 64; ++w) {
+            tot += s_sum[v][w];
+            vis += s_vis[v][w] & ~kVisRefused;
+            flag |= s_vis[v][w] & kVisRefused;
+        }
+        // per block, so that the frame-info words need no clearing before
+        // the preprocess: the block scan sums the counts and flags
         a.vb[v].block_sums[blockIdx.x] = tot;
-        if (vis) atomicAdd(&a.vb[v].frame_info[FI_N_VISIBLE], vis);
+        a.vb[v].block_vis[blockIdx.x] = vis | flag;
     }
 }
 
@@ -597,10 +604,10 @@ __global__ __launch_bounds__(kPrimBlock) void preprocess_kernel(PreprocessArgs g
     for (int v = 0; v < NV; ++v) {
         const bool refused = touched[v] == kTouchedRefused;
         if (refused) touched[v] = 0u;
-        if (__ballot(refused) != 0ull && lane == 0) atomicOr(&a.vb[v].frame_info[FI_ERROR], 8u);
+        const uint32_t refused_w = __ballot(refused) != 0ull ? kVisRefused : 0u;
         if (valid) a.vb[v].n_touched[i] = touched[v];
         const uint32_t ws = wave_sum(touched[v]);
-        const uint32_t wv = (uint32_t)__popcll(__ballot(visible[v]));
+        const uint32_t wv = (uint32_t)__popcll(__ballot(visible[v])) | refused_w;
         if (lane == 0) { s_sum[v][wid] = ws; s_vis[v][wid] = wv; }
     }
     __syncthreads();
@@ -609,9 +616,16 @@ __global__ __launch_bounds__(kPrimBlock) void preprocess_kernel(PreprocessArgs g
     for (int v = 0; v < NV; ++v) {
         if ((int)threadIdx.x != v) continue;
         uint32_t tot = 0, vis = 0;
-        for (int w = 0; w < kPrimBlock / 64; ++w) { tot += s_sum[v][w]; vis += s_vis[v][w]; }
+        uint32_t flag = 0;
+        for (int w = 0; w < kPrimBlock / 64; ++w) {
+            tot += s_sum[v][w];
+            vis += s_vis[v][w] & ~kVisRefused;
+            flag |= s_vis[v][w] & kVisRefused;
+        }
+        // per block, so that the frame-info words need no clearing before
+        // the preprocess: the block scan sums the counts and flags
         a.vb[v].block_sums[blockIdx.x] = tot;
-        if (vis) atomicAdd(&a.vb[v].frame_info[FI_N_VISIBLE], vis);
+        a.vb[v].block_vis[blockIdx.x] = vis | flag;
     }
 }
 
