@@ -235,6 +235,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--view-branches", type=int, default=None,
                     help="concurrent view branches of a frame (library default 2; 1 = views one after another)")
+    ap.add_argument("--work-tile", type=int, choices=(1, 2, 4), default=None,
+                    help="blend work tile as waves per workgroup: 1 = 8x8 (default), 2 = 16x8, 4 = 16x16")
     ap.add_argument("--xcd-run", type=int, default=None,
                     help="blend tiles per XCD run (library default when omitted)")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-view-per-launch comparison")
@@ -290,6 +292,9 @@ def main() -> int:
     handle.set_option(_lib.OPT_STAGE_TIMING, 0)
     if args.xcd_run is not None:
         handle.set_option(_lib.OPT_BLEND_XCD_RUN, args.xcd_run)
+    if args.work_tile is not None:
+        handle.set_option(_lib.OPT_BLEND_WORK_TILE, args.work_tile)
+    work_tile = handle.tuned_work_tile(args.width, args.height)  # the one the timed frames use
     if args.view_branches is not None:
         handle.set_option(_lib.OPT_VIEW_BRANCHES, args.view_branches)
     torch.cuda.synchronize()
@@ -474,7 +479,8 @@ def main() -> int:
                                 + ", shared preprocess"),
                    "gaussians": args.gaussians, "width": args.width, "height": args.height, "K": args.K,
                    "fov_deg": args.fov, "foveation_radius_px": fr, "blend_width_px": bw,
-                   "quarter_radius_px": qr, "parallelism": "single GPU"},
+                   "quarter_radius_px": qr, "parallelism": "single GPU",
+                   "blend_work_tile": {1: "8x8", 2: "16x8", 4: "16x16"}[work_tile]},
         "p50_ms": percentile(lat, 50),
         "p99_ms": percentile(lat, 99),
         "latency_frames": args.latency_frames,

@@ -37,6 +37,7 @@ OPT_STAGE_TIMING = 3
 OPT_BLEND_XCD_RUN = 4
 OPT_FRAME_MODE = 5
 OPT_VIEW_BRANCHES = 6
+OPT_BLEND_WORK_TILE = 7  # waves per blend workgroup: 1 (8x8), 2 (16x8), 4 (16x16)
 FRAME_EXACT, FRAME_DEVICE, FRAME_GRAPH = 0, 1, 2
 NUM_STAGES = 11
 STAGE_NAMES = ("clear", "tiles", "preprocess", "scan", "sync", "instances", "sort", "ranges",
@@ -46,7 +47,8 @@ EXPORTED_SYMBOLS = (
     "fhtgs_create", "fhtgs_destroy", "fhtgs_last_error", "fhtgs_version", "fhtgs_render",
     "fhtgs_workspace_bytes", "fhtgs_workspace_query", "fhtgs_sort_pairs", "fhtgs_fovea_radius_tiles",
     "fhtgs_debug_read", "fhtgs_debug_counts", "fhtgs_set_option", "fhtgs_stage_times",
-    "fhtgs_render_batch", "fhtgs_autotune", "fhtgs_tuned_digit_bits", "fhtgs_get_option",
+    "fhtgs_render_batch", "fhtgs_autotune", "fhtgs_tuned_digit_bits", "fhtgs_tuned_work_tile",
+    "fhtgs_get_option",
     "fhtgs_visualize_gaze", "fhtgs_visualize_tile_boundaries", "fhtgs_build_tables",
     "fhtgs_validate_view", "fhtgs_sort_pairs_noalloc",
 )
@@ -121,6 +123,8 @@ def load() -> ctypes.CDLL:
         lib.fhtgs_autotune.restype = ctypes.c_int
         lib.fhtgs_tuned_digit_bits.argtypes = [vp, i32, i32, ctypes.POINTER(i32)]
         lib.fhtgs_tuned_digit_bits.restype = ctypes.c_int
+        lib.fhtgs_tuned_work_tile.argtypes = [vp, i32, i32, ctypes.POINTER(i32)]
+        lib.fhtgs_tuned_work_tile.restype = ctypes.c_int
         lib.fhtgs_get_option.argtypes = [vp, i32, ctypes.POINTER(i64)]
         lib.fhtgs_get_option.restype = ctypes.c_int
         f32 = ctypes.c_float
@@ -290,6 +294,13 @@ class Handle:
         b = ctypes.c_int32()
         self.check(self.lib.fhtgs_tuned_digit_bits(self.ptr, int(width), int(height), ctypes.byref(b)),
                    "fhtgs_tuned_digit_bits")
+        return b.value
+
+    def tuned_work_tile(self, width: int, height: int) -> int:
+        """Waves per blend workgroup the handle uses for this image size: 1 (8x8), 2 (16x8), 4 (16x16)."""
+        b = ctypes.c_int32()
+        self.check(self.lib.fhtgs_tuned_work_tile(self.ptr, int(width), int(height), ctypes.byref(b)),
+                   "fhtgs_tuned_work_tile")
         return b.value
 
     def sort_pairs(self, keys_ptr: int, vals_ptr: int, n: int, key_bytes: int, end_bit: int,

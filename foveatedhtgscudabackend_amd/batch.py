@@ -75,13 +75,16 @@ def render_batch(scenes: list, views: list) -> list:
 
 
 def autotune(scenes: list, views: list, frames: int = 3) -> dict:
-    """Tunes the handle for this batch (sort digit width per view size, blend
-    XCD run, concurrent view branches; images are unchanged by all three) and
-    returns the choices."""
+    """Tunes the handle for this batch (per view size the sort digit width and
+    the blend work tile, 8x8 / 16x8 / 16x16; for the batch the blend XCD run
+    and the concurrent view branches; images are unchanged by all of them)
+    and returns the choices."""
     handle, structs, vstructs, index, slots, images, stream, keep = _marshal(scenes, views)
     handle.autotune(structs, vstructs, index, stream, frames, slots)
     del keep
     sizes = sorted({(bv.settings.width, bv.settings.height) for bv in views})
+    names = {1: "8x8", 2: "16x8", 4: "16x16"}
     return {"digit_bits": {f"{w}x{h}": handle.tuned_digit_bits(w, h) for w, h in sizes},
+            "work_tile": {f"{w}x{h}": names[handle.tuned_work_tile(w, h)] for w, h in sizes},
             "blend_xcd_run": handle.get_option(_lib.OPT_BLEND_XCD_RUN),
             "view_branches": handle.get_option(_lib.OPT_VIEW_BRANCHES)}

@@ -125,6 +125,7 @@ struct fhtgs_context {
     int keep_unsorted = 0;
     int digit_bits = 11;
     int xcd_run = 16;
+    int work_tile = 1;  // waves per blend workgroup: 1 (8x8), 2 (16x8) or 4 (16x16 work tiles)
     int timing = 0;
     int frame_mode = kModeGraph;
     int last_mode = kModeExact;
@@ -151,11 +152,20 @@ struct fhtgs_context {
     uint64_t graph_clock = 0;
     // autotuned sort digit width per view size (n_small); others use digit_bits
     std::vector<std::pair<int64_t, int>> tuned_bits;
+    // autotuned blend work tile per view size (n_small); others use work_tile
+    std::vector<std::pair<int64_t, int>> tuned_wt;
     // tile-boundary overlay: its own parameters and tile buffers
     ViewSlot vis;
     DeviceBuffer vis_params;
     ViewParams* vis_params_host = nullptr;
 };
+
+// The blend work tile of a view size (autotuned, or the handle's default).
+static int work_tile_for(const fhtgs_context* h, int64_t n_small) {
+    for (const auto& t : h->tuned_wt)
+        if (t.first == n_small) return t.second;
+    return h->work_tile;
+}
 
 namespace {
 
@@ -405,6 +415,7 @@ std::vector<int64_t> launch_signature(const fhtgs_context* h, const std::vector<
         g.push_back(sl.ib.capacity);
         g.push_back(sl.plan.passes);
         g.push_back(sl.plan.bits[0]);
+        g.push_back(work_tile_for(h, sl.vp.n_small));
         g.push_back((int64_t)(uintptr_t)sl.fixed.ptr);
         g.push_back((int64_t)(uintptr_t)sl.inst.ptr);
     }
@@ -552,13 +563,13 @@ int enqueue_ranges(fhtgs_context* h, ViewSlot& slot, bool exact, hipStream_t s) 
 
 int enqueue_blend(fhtgs_context* h, ViewSlot& slot, const fhtgs_view& view, bool exact, hipStream_t s) {
     if (view.clear_output) HCHK(launch_clear_image(slot.vp, slot.dvp, slot.vb, s));
-    uint32_t grid = (uint32_t)slot.vp.n_small;
-    if (exact) {  // QUARTER tiles (the last partition) go four to a workgroup
-        const uint32_t nq = slot.host_fi[FI_PART + 7] - slot.host_fi[FI_PART + 6];
-        grid = slot.n_active - nq + (nq + 3u) / 4u;
-    }
-    HCHK(launch_blend_order(slot.vb, slot.vp.n_small, s));
-    HCHK(launch_blend(slot.dvp, slot.vb, slot.ib.vals[slot.sorted_src], view.K, grid, h->xcd_run, s));
+    const int wt = work_tile_for(h, slot.vp.n_small);
+    // a bound on the work units: n_small single-wave units, or n_small / W + 2
+    uint32_t grid = (uint32_t)(wt <= 1 ? slot.vp.n_small : slot.vp.n_small / wt + 2);
+    if (exact) grid = blend_units(slot.host_fi + FI_PART, slot.n_active, wt);
+    HCHK(launch_blend_order(slot.vb, slot.vp.n_small, wt, s));
+    HCHK(launch_blend(slot.dvp, slot.vb, slot.ib.vals[slot.sorted_src], view.K, grid, h->xcd_run, wt,
+                      (uint32_t)slot.vp.n_small, s));
     return FHTGS_OK;
 }
 
@@ -617,6 +628,7 @@ int digit_bits_for(const fhtgs_context* h, int64_t n_small) {
         if (t.first == n_small) return t.second;
     return h->digit_bits;
 }
+
 
 int size_instances(fhtgs_context* h, ViewSlot& slot, int64_t cap) {
     const int end_bit = end_bit_of((uint32_t)slot.vp.n_small + 1u);
@@ -952,6 +964,12 @@ int fhtgs_set_option(fhtgs_handle h, int32_t option, int64_t value) {
             if (value < 0 || value > 4096) return set_err(h, FHTGS_E_INVALID, "XCD run must be 0..4096");
             h->xcd_run = (int)value;
             return FHTGS_OK;
+        case FHTGS_OPT_BLEND_WORK_TILE:
+            if (value != 1 && value != 2 && value != 4)
+                return set_err(h, FHTGS_E_INVALID, "blend work tile must be 1 (8x8), 2 (16x8) or 4 (16x16)");
+            h->work_tile = (int)value;
+            h->tuned_wt.clear();
+            return FHTGS_OK;
         default: return set_err(h, FHTGS_E_INVALID, "unknown option");
     }
 }
@@ -1161,6 +1179,22 @@ int fhtgs_autotune(fhtgs_handle h, const fhtgs_scene* scenes, int32_t n_scenes, 
         if (rc) break;
         for (auto& t : h->tuned_bits)
             if (t.first == ns) t.second = best_bits;
+        // the blend's work tile for this size: 8x8, 16x8 or 16x16 (waves per
+        // workgroup 1, 2, 4); every pixel sees the same Gaussians in the same
+        // order with each
+        int best_wt = work_tile_for(h, ns);
+        best = 1e30f;
+        for (int wt : {1, 2, 4}) {
+            auto it = std::find_if(h->tuned_wt.begin(), h->tuned_wt.end(), [&](const std::pair<int64_t, int>& t) { return t.first == ns; });
+            if (it == h->tuned_wt.end()) h->tuned_wt.push_back({ns, wt});
+            else it->second = wt;
+            float ms = 0.0f;
+            if ((rc = timed(sel, ms))) break;
+            if (ms < best) { best = ms; best_wt = wt; }
+        }
+        for (auto& t : h->tuned_wt)
+            if (t.first == ns) t.second = best_wt;
+        if (rc) break;
     }
     std::vector<int32_t> all;
     for (int v = 0; v < n_views; ++v) all.push_back(v);
@@ -1205,6 +1239,7 @@ int fhtgs_get_option(fhtgs_handle h, int32_t option, int64_t* value) {
         case FHTGS_OPT_BLEND_XCD_RUN: *value = h->xcd_run; return FHTGS_OK;
         case FHTGS_OPT_FRAME_MODE: *value = h->frame_mode; return FHTGS_OK;
         case FHTGS_OPT_VIEW_BRANCHES: *value = h->branches; return FHTGS_OK;
+        case FHTGS_OPT_BLEND_WORK_TILE: *value = h->work_tile; return FHTGS_OK;
         default: return set_err(h, FHTGS_E_INVALID, "unknown option");
     }
 }
@@ -1213,6 +1248,13 @@ int fhtgs_tuned_digit_bits(fhtgs_handle h, int32_t width, int32_t height, int32_
     if (!h || !bits || width <= 0 || height <= 0) return FHTGS_E_INVALID;
     const int64_t ns = (int64_t)((width + kTileL - 1) / kTileL) * ((height + kTileL - 1) / kTileL) * kNumSub;
     *bits = digit_bits_for(h, ns);
+    return FHTGS_OK;
+}
+
+int fhtgs_tuned_work_tile(fhtgs_handle h, int32_t width, int32_t height, int32_t* waves) {
+    if (!h || !waves || width <= 0 || height <= 0) return FHTGS_E_INVALID;
+    const int64_t ns = (int64_t)((width + kTileL - 1) / kTileL) * ((height + kTileL - 1) / kTileL) * kNumSub;
+    *waves = work_tile_for(h, ns);
     return FHTGS_OK;
 }
 
@@ -1326,12 +1368,10 @@ int fhtgs_debug_read(fhtgs_handle h, int32_t view, int32_t id, void* dst, int64_
         case FHTGS_DBG_TILE_MAP_PART: src = sl.vb.tile_map_part; bytes = na * 4; break;
         case FHTGS_DBG_PARTITION: src = sl.vb.frame_info + FI_PART; bytes = 24; break;
         case FHTGS_DBG_PARTITION_QUARTER: src = sl.vb.frame_info + FI_PART + 6; bytes = 8; break;
-        case FHTGS_DBG_BLEND_ORDER: {
-            const int64_t nq = (int64_t)sl.host_fi[FI_PART + 7] - sl.host_fi[FI_PART + 6];
+        case FHTGS_DBG_BLEND_ORDER:
             src = sl.vb.blend_order;
-            bytes = (na - nq + (nq + 3) / 4) * 4;
+            bytes = (int64_t)blend_units(sl.host_fi + FI_PART, (uint32_t)na, work_tile_for(h, sl.vp.n_small)) * 4;
             break;
-        }
         case FHTGS_DBG_N_TOUCHED: src = sl.vb.n_touched; bytes = N * 4; break;
         case FHTGS_DBG_BOUNDS: src = sl.vb.bounds; bytes = N * 8; break;
         case FHTGS_DBG_VPMT1: src = sl.vb.prim; bytes = N * 16; field = 0; break;

@@ -170,9 +170,14 @@ hipError_t launch_sort_histogram(const InstanceBuffers& ib, const SortPlan& plan
                                  uint32_t n, hipStream_t s);
 hipError_t launch_ranges(const ViewBuffers& vb, const InstanceBuffers& ib, const SortPlan& plan,
                          int src, uint32_t n_grid, hipStream_t s);
+// work_tile: waves per blend workgroup, 1 (8x8 work tiles), 2 (16x8) or 4
+// (16x16); the work order and the blend launch must use the same value.
 hipError_t launch_blend(const ViewParams* dvp, const ViewBuffers& vb, const uint32_t* sorted_vals,
-                        int K, uint32_t grid, int xcd_run, hipStream_t s);
-hipError_t launch_blend_order(const ViewBuffers& vb, int n_small, hipStream_t s);
+                        int K, uint32_t grid, int xcd_run, int work_tile, uint32_t n_small, hipStream_t s);
+hipError_t launch_blend_order(const ViewBuffers& vb, int n_small, int work_tile, hipStream_t s);
+// Blend work units of a frame (the grid of an exact launch) from its
+// partition ranges (FI_PART words) and active tile count.
+uint32_t blend_units(const uint32_t* part, uint32_t n_active, int work_tile);
 hipError_t launch_upsample(const ViewParams* dvp, const ViewBuffers& vb, uint32_t grid, hipStream_t s);
 hipError_t launch_clear_image(const ViewParams& vp, const ViewParams* dvp, const ViewBuffers& vb,
                               hipStream_t s);

@@ -22,7 +22,10 @@
  *   fhtgs_render_batch        <- no reference counterpart: many views of several
  *                                resident scenes per call (the reference renders one
  *                                view or one stereo pair per call)
- *   fhtgs_autotune, fhtgs_tuned_digit_bits, fhtgs_get_option: no reference counterpart
+ *   fhtgs_autotune, fhtgs_tuned_digit_bits, fhtgs_tuned_work_tile, fhtgs_get_option:
+ *                                no reference counterpart (the reference fixes its tile
+ *                                sizes and digit width at compile time,
+ *                                rasterization/include/config.h:21-27)
  *   fhtgs_visualize_gaze      <- htgs_foveated::visualization::visualize_gaze_wrapper,
  *                                visualization/include/visualization_api.h:8-14,
  *                                visualization/src/visualization_api.cu:13-35
@@ -176,15 +179,16 @@ int fhtgs_render_batch(fhtgs_handle h, const fhtgs_scene* scenes, int32_t n_scen
 
 /* Autotuner for a batch (no reference counterpart): renders it once exactly,
  * then times device-count frames (best of `frames`) to pick, per view size,
- * the sort's digit width (8 or 11 bits) and, for the batch, the blend's XCD
- * run (0, 16 or 64) and the number of concurrent view branches (2 to 4,
+ * the sort's digit width (8 or 11 bits) and the blend's work tile (8x8, 16x8
+ * or 16x16 pixels rendered by one workgroup from one staged list,
+ * FHTGS_OPT_BLEND_WORK_TILE), and, for the batch, the blend's XCD run (0, 16
+ * or 64) and the number of concurrent view branches (2 to 4,
  * FHTGS_OPT_VIEW_BRANCHES); the choices stay in the handle. Only parameters
  * that leave every image unchanged are tuned: K changes which fragments
  * enter the core, and the 16-px tiles define the foveation classes and the
- * sample grid. The blend's work tile (which pixels one workgroup renders
- * from one staged list) would leave images unchanged but is fixed at 8x8:
- * wider work tiles that share one staging measured as no gain (see the
- * audit's --waves report). Leaves the batch's images rendered. */
+ * sample grid. A 32x8 work tile would span two 16-px tiles whose instance
+ * lists differ, so it would share no staging and is not offered. Leaves the
+ * batch's images rendered. */
 int fhtgs_autotune(fhtgs_handle h, const fhtgs_scene* scenes, int32_t n_scenes,
                    const fhtgs_view* views, const int32_t* view_scene, const int32_t* view_slot,
                    int32_t n_views, int32_t frames, void* stream);
@@ -192,6 +196,11 @@ int fhtgs_autotune(fhtgs_handle h, const fhtgs_scene* scenes, int32_t n_scenes,
 /* The digit width the handle's sort uses for an image size (tuned or the
  * FHTGS_OPT_SORT_DIGIT_BITS default). */
 int fhtgs_tuned_digit_bits(fhtgs_handle h, int32_t width, int32_t height, int32_t* bits);
+
+/* The blend work tile the handle uses for an image size, as waves per
+ * workgroup: 1 (8x8), 2 (16x8) or 4 (16x16) (tuned or the
+ * FHTGS_OPT_BLEND_WORK_TILE default). */
+int fhtgs_tuned_work_tile(fhtgs_handle h, int32_t width, int32_t height, int32_t* waves);
 
 /* Device bytes one view slot needs, before any render (no handle, no
  * device call): *fixed_bytes for the per-Gaussian and per-tile buffers of
@@ -330,6 +339,12 @@ int fhtgs_debug_counts(fhtgs_handle h, int32_t view, int64_t* counts);
  * classification overlaps the preprocess; 1 = one stream. Values 1..4,
  * default 2. Images do not depend on it. */
 #define FHTGS_OPT_VIEW_BRANCHES 6
+/* Blend work tile, as waves per workgroup: 1 = 8x8 (one wave per 8x8 tile,
+ * default), 2 = 16x8, 4 = 16x16: the sub-tiles of a FOVEA or BLENDED 16-px
+ * tile share one staging of their (identical) instance list, and PERIPHERY
+ * and QUARTER tiles go 2 or 4 waves to a workgroup. Setting it clears the
+ * autotuned choices. Images do not depend on it. */
+#define FHTGS_OPT_BLEND_WORK_TILE 7
 int fhtgs_set_option(fhtgs_handle h, int32_t option, int64_t value);
 int fhtgs_get_option(fhtgs_handle h, int32_t option, int64_t* value);
 

@@ -12,7 +12,8 @@
 // that instruction's error, so its masks are subsets of the kernel's. Input is a binary dump of one oracle frame written
 // by tests/test_cull.py; prints pair counts and the number of violations of
 // each (both must be 0), and the smallest slack, in pixels, between a reached
-// sample and its row interval's edge.
+// sample and its row interval's edge. The same check covers the masks of the
+// 16x8 and 16x16 work tiles (work_tile_mask) on FOVEA and BLENDED tiles.
 //
 // With --waves it prints instead the blend's wave-level loop counts for the
 // frame, as the kernel runs it (batches of 64 staged Gaussians per 8x8 wave,
@@ -72,6 +73,27 @@ static uint64_t sample_mask(const float rec[5], int bx, int by, int step, int si
         uc_out[r] = uc;
         lo[r] = w < 0.0f ? NAN : uc - w;
         hi[r] = w < 0.0f ? NAN : uc + w;
+    }
+    return m;
+}
+
+// The mask of sub-tile (sx, sy) of a 16-column work tile whose first pixel
+// is (wx, wy), with `rows` pixel rows (8: 16x8, 16: 16x16), as the wide blend
+// computes it: one 16-column row interval per pixel row of the work tile
+// (fh_mask_row_bits with 16 columns), its low 8 columns the left sub-tile's,
+// its high 8 the right one's; bit r * 8 + c for the sub-tile's sample (c, r).
+// Square roots lowered as in sample_mask.
+static uint64_t work_tile_mask(const float rec[5], int wx, int wy, int rows, int sx, int sy) {
+    const FhMaskPrep p = fh_mask_prep(rec, 1.0f / rec[3], (float)wx, (float)wy, 1.0f);
+    if (p.all) return ~0ull;
+    uint64_t m = 0;
+    for (int r = 0; r < rows; ++r) {
+        if (r / 8 != sy) continue;
+        const float dy = (float)r + p.dy0;
+        const float w2 = fh_mask_row_w2(p, dy);
+        const float w = w2 < 0.0f ? -1.0f : sqrtf(w2) * (1.0f - 0x1p-20f) + kMaskMarginPx;
+        const uint32_t bits = fh_mask_row_bits(p, dy, w, 16);
+        m |= (uint64_t)((bits >> (8 * sx)) & 0xffu) << ((r % 8) * 8);
     }
     return m;
 }
@@ -262,6 +284,9 @@ int main(int argc, char** argv) {
     }
     int64_t pairs = 0, kept = 0, reach = 0, bad = 0;
     int64_t px_reach = 0, px_pre = 0, px_bad = 0, px_evals = 0, no_conic = 0;
+    // FOVEA / BLENDED samples a Gaussian reaches, and those missing from its
+    // 16x8 or 16x16 work-tile mask
+    int64_t wt_reach = 0, wt_bad = 0, wt_bits8 = 0, wt_bits16 = 0;
     double min_slack = 1e30;
     for (int type = 0; type < 4; ++type) {
         // periphery tiles sample their large tile at stride 2 (8x8 samples),
@@ -282,6 +307,15 @@ int main(int argc, char** argv) {
                 float lo[8], hi[8], uc[8];
                 const uint64_t m = sample_mask(rec, bx, by, step, side, &all, lo, hi, uc);
                 const bool keep = m != 0;
+                const bool wide = type == 0 || type == 2;
+                uint64_t mw8 = 0, mw16 = 0;
+                if (wide) {
+                    const int wx = (int)(lt % gw) * 16, sx = (int)(st % 2), sy = (int)(st / 2);
+                    mw8 = work_tile_mask(rec, wx, (int)(lt / gw) * 16 + 8 * sy, 8, sx, 0);
+                    mw16 = work_tile_mask(rec, wx, (int)(lt / gw) * 16, 16, sx, sy);
+                    wt_bits8 += __builtin_popcountll(mw8);
+                    wt_bits16 += __builtin_popcountll(mw16);
+                }
                 bool any = false;
                 for (int q = 0; q < side * side && !any; ++q) {
                     const int px = bx + (q % side) * step, py = by + (q / side) * step;
@@ -306,6 +340,14 @@ int main(int argc, char** argv) {
                             const double sl = std::min((double)c - (double)lo[rr], (double)hi[rr] - (double)c) * step;
                             if (sl < min_slack) min_slack = sl;
                         }
+                        if (wide && r) {
+                            ++wt_reach;
+                            if (!((mw8 >> q) & 1ull) || !((mw16 >> q) & 1ull)) {
+                                if (wt_bad < 5)
+                                    std::printf("work-tile violation: tile %u prim %u px %d py %d\n", tg, p, px, py);
+                                ++wt_bad;
+                            }
+                        }
                         if (r && !pt) {
                             if (px_bad < 5)
                                 std::printf("pixel violation: tile %u prim %u px %d py %d rec=(%a %a %a %a %a)\n", tg,
@@ -328,5 +370,7 @@ int main(int argc, char** argv) {
     std::printf("pixels %lld reached %lld pretest %lld pixel_violations %lld no_conic %lld min_slack %.6g\n",
                 (long long)px_evals, (long long)px_reach, (long long)px_pre, (long long)px_bad, (long long)no_conic,
                 min_slack);
-    return bad != 0 || px_bad != 0;
+    std::printf("work_tiles reached %lld violations %lld mask_bits_16x8 %lld mask_bits_16x16 %lld\n",
+                (long long)wt_reach, (long long)wt_bad, (long long)wt_bits8, (long long)wt_bits16);
+    return bad != 0 || px_bad != 0 || wt_bad != 0;
 }

@@ -76,3 +76,10 @@ def test_cull_drops_only_unreachable(case, oracle_lib, audit_exe, tmp_path):
     assert pbad == 0, r.stdout
     assert reached <= pre <= evals and reached > 0
     assert float(m.group(6)) > 0.0
+    # the 16x8 and 16x16 work tiles' masks (one 16-column interval per row for
+    # two sub-tiles) keep every FOVEA / BLENDED sample a Gaussian reaches
+    m = re.search(r"work_tiles reached (\d+) violations (\d+)", r.stdout)
+    assert m, r.stdout
+    assert int(m.group(2)) == 0, r.stdout
+    if case.foveation_radius_px != -1:
+        assert int(m.group(1)) > 0, "the case has FOVEA or BLENDED tiles"

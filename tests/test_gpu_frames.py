@@ -241,6 +241,8 @@ def test_slots_and_autotune_leave_images_unchanged(restore_mode):
         print(choice)
         assert set(choice["digit_bits"].values()) <= {8, 11}
         assert choice["blend_xcd_run"] in (0, 16, 64)
+        assert set(choice["work_tile"].values()) <= {"8x8", "16x8", "16x16"}
+        assert set(choice["work_tile"]) == set(choice["digit_bits"])
         for rep in range(3):
             out = render_batch(scenes, views)
             torch.cuda.synchronize()
@@ -252,6 +254,7 @@ def test_slots_and_autotune_leave_images_unchanged(restore_mode):
     finally:
         h.set_option(lib.OPT_SORT_DIGIT_BITS, bits0)
         h.set_option(lib.OPT_BLEND_XCD_RUN, 16)
+        h.set_option(lib.OPT_BLEND_WORK_TILE, 1)  # also clears the tuned work tiles
 
 
 @pytest.mark.parametrize("n_visible", [1, 0])

@@ -31,6 +31,7 @@ def fresh_handle_options():
     h.set_option(_lib.OPT_FRAME_MODE, _lib.FRAME_GRAPH)
     h.set_option(_lib.OPT_SORT_DIGIT_BITS, bits)
     h.set_option(_lib.OPT_BLEND_XCD_RUN, run)
+    h.set_option(_lib.OPT_BLEND_WORK_TILE, 1)
 
 
 def test_batched_workload_equals_views_rendered_alone(fresh_handle_options):
@@ -48,6 +49,8 @@ def test_batched_workload_equals_views_rendered_alone(fresh_handle_options):
         out = render_batch(scenes, views)
     torch.cuda.synchronize()
     h.set_option(lib.OPT_FRAME_MODE, lib.FRAME_EXACT)
+    # the views alone with 8x8 work tiles, whatever work tiles the batch was tuned to
+    h.set_option(lib.OPT_BLEND_WORK_TILE, 1)
     rast = {}
     for k, bv in enumerate(views):
         key = (bv.foveation_radius_px, bv.blend_width_px, bv.quarter_radius_px)
