@@ -644,18 +644,25 @@ int size_instances(fhtgs_context* h, ViewSlot& slot, int64_t cap) {
 int render_impl(fhtgs_context* h, const fhtgs_scene* scenes, int n_scenes, const fhtgs_view* views,
                 const int32_t* view_scene, const int32_t* view_slot, int n_views, bool pair_views,
                 hipStream_t s, int force_mode) {
-    // ---- preprocess groups: consecutive views of one scene that agree on
-    // the preprocess parameters share one pass
+    // ---- preprocess groups: two views of one scene that agree on the
+    // preprocess parameters share one pass (each view's outputs depend on
+    // that view alone), whether or not they are adjacent in the call: each
+    // view is paired with the next unpaired one that qualifies
     std::vector<PreGroup> groups;
-    for (int v = 0; v < n_views;) {
+    std::vector<char> grouped((size_t)n_views, 0);
+    for (int v = 0; v < n_views; ++v) {
+        if (grouped[(size_t)v]) continue;
         PreGroup pg{view_scene[v], 1, {v, v}};
-        if (pair_views && v + 1 < n_views && view_scene[v + 1] == view_scene[v] &&
-            same_preprocess(views[v], views[v + 1])) {
-            pg.nv = 2;
-            pg.v[1] = v + 1;
+        grouped[(size_t)v] = 1;
+        for (int w = v + 1; pair_views && w < n_views; ++w) {
+            if (!grouped[(size_t)w] && view_scene[w] == view_scene[v] && same_preprocess(views[v], views[w])) {
+                pg.nv = 2;
+                pg.v[1] = w;
+                grouped[(size_t)w] = 1;
+                break;
+            }
         }
         groups.push_back(pg);
-        v += pg.nv;
     }
     // view v renders into slot view_slot[v] (default v): a slot keeps its
     // buffers and instance capacity from call to call
