@@ -2,9 +2,9 @@
 
 The reference renders one view (``_C.render``) or one stereo pair
 (``_C_stereo.render``) per call. ``render_batch`` hands a whole batch to
-``fhtgs_render_batch``: two views of one scene that agree on the preprocess
-parameters share one read of the Gaussians (stereo pairs, or any two such
-views of the batch), the
+``fhtgs_render_batch``: up to four views of one scene that agree on the
+preprocess parameters share one read of the Gaussians (stereo pairs, or any
+such views of the batch), the
 frame is enqueued without a mid-frame host wait, and repeated batches of the
 same shape replay one captured hipGraph. Every image equals what
 ``FoveatedHTGSRasterizer.render`` returns for that view alone.

@@ -96,10 +96,13 @@ struct ViewSlot {
 };
 
 // One preprocess launch: a scene and one or two of its views.
+// Views whose Gaussians one preprocess pass reads once: one scene, equal
+// preprocess parameters, at most kMaxGroup views.
+constexpr int kMaxGroup = 4;
 struct PreGroup {
     int scene;
     int nv;
-    int v[2];
+    int v[kMaxGroup];
 };
 
 enum FrameMode { kModeExact = 0, kModeDevice = 1, kModeGraph = 2 };
@@ -398,8 +401,7 @@ std::vector<int64_t> launch_signature(const fhtgs_context* h, const std::vector<
     for (const PreGroup& pg : groups) {
         g.push_back(pg.scene);
         g.push_back(pg.nv);
-        g.push_back(pg.v[0]);
-        g.push_back(pg.v[1]);
+        for (int k = 0; k < kMaxGroup; ++k) g.push_back(k < pg.nv ? pg.v[k] : -1);
         g.push_back(scenes[pg.scene].n_primitives);
         g.push_back(scenes[pg.scene].total_sh_bases);
         g.push_back(views[pg.v[0]].anti_aliasing ? 1 : 0);
@@ -482,8 +484,12 @@ int enqueue_phase1(fhtgs_context* h, const std::vector<PreGroup>& groups, const 
     };
     auto preprocess = [&]() -> int {
         for (const PreGroup& pg : groups) {
-            const ViewParams* hv[2] = {&(*h->cur[pg.v[0]]).vp, &(*h->cur[pg.v[pg.nv - 1]]).vp};
-            const ViewBuffers vbs[2] = {(*h->cur[pg.v[0]]).vb, (*h->cur[pg.v[pg.nv - 1]]).vb};
+            const ViewParams* hv[kMaxGroup];
+            ViewBuffers vbs[kMaxGroup];
+            for (int k = 0; k < pg.nv; ++k) {
+                hv[k] = &(*h->cur[pg.v[k]]).vp;
+                vbs[k] = (*h->cur[pg.v[k]]).vb;
+            }
             const fhtgs_scene& sc = scenes[pg.scene];
             const SceneArgs sa{sc.positions, sc.scales, sc.rotations, sc.opacities, sc.sh_0, sc.sh_rest};
             HCHK(launch_preprocess(hv, vbs, pg.nv, sa, (uint32_t)sc.n_primitives, s));
@@ -644,22 +650,20 @@ int size_instances(fhtgs_context* h, ViewSlot& slot, int64_t cap) {
 int render_impl(fhtgs_context* h, const fhtgs_scene* scenes, int n_scenes, const fhtgs_view* views,
                 const int32_t* view_scene, const int32_t* view_slot, int n_views, bool pair_views,
                 hipStream_t s, int force_mode) {
-    // ---- preprocess groups: two views of one scene that agree on the
-    // preprocess parameters share one pass (each view's outputs depend on
-    // that view alone), whether or not they are adjacent in the call: each
-    // view is paired with the next unpaired one that qualifies
+    // ---- preprocess groups: up to kMaxGroup views of one scene that agree
+    // on the preprocess parameters share one pass (each view's outputs depend
+    // on that view alone), whether or not they are adjacent in the call: each
+    // view is grouped with the next ungrouped ones that qualify
     std::vector<PreGroup> groups;
     std::vector<char> grouped((size_t)n_views, 0);
     for (int v = 0; v < n_views; ++v) {
         if (grouped[(size_t)v]) continue;
-        PreGroup pg{view_scene[v], 1, {v, v}};
+        PreGroup pg{view_scene[v], 1, {v, v, v, v}};
         grouped[(size_t)v] = 1;
-        for (int w = v + 1; pair_views && w < n_views; ++w) {
+        for (int w = v + 1; pair_views && w < n_views && pg.nv < kMaxGroup; ++w) {
             if (!grouped[(size_t)w] && view_scene[w] == view_scene[v] && same_preprocess(views[v], views[w])) {
-                pg.nv = 2;
-                pg.v[1] = w;
+                pg.v[pg.nv++] = w;
                 grouped[(size_t)w] = 1;
-                break;
             }
         }
         groups.push_back(pg);

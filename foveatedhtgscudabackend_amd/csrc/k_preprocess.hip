@@ -16,28 +16,30 @@
 
 namespace fhtgs {
 
-constexpr int kMaxViewsPerPass = 2;
+constexpr int kMaxViewsPerPass = 4;
 // LDS slots of SH coefficients per wave (a visible Gaussian each; a wave
 // with more visible Gaussians copies the rest in a further round). 44 slots
 // of 45 floats are 31.7 KB per workgroup of four waves: the most that still
 // lets the five workgroups per CU which the registers allow share the LDS.
 constexpr int kShSlots = 44;
 
-// Kernel argument: the views' parameters and the scene's tensors by value
+// Kernel argument: the NV views' parameters and the scene's tensors by value
 // (pointers read from kernel arguments are global ones to the compiler). The
 // preprocess therefore does not wait for the frame's parameter upload, which
 // only the later kernels read.
+template <int NV>
 struct PreprocessArgs {
-    ViewParams vp[kMaxViewsPerPass];
-    ViewBuffers vb[kMaxViewsPerPass];
+    ViewParams vp[NV];
+    ViewBuffers vb[NV];
     SceneArgs scene;
     uint32_t n;
 };
 
 // The same, resolved into registers at kernel start.
+template <int NV>
 struct PreprocessState {
-    ViewParams vp[kMaxViewsPerPass];
-    ViewBuffers vb[kMaxViewsPerPass];
+    ViewParams vp[NV];
+    ViewBuffers vb[NV];
     const float* pos;
     const float* scl;
     const float* rot;
@@ -48,8 +50,8 @@ struct PreprocessState {
 };
 
 template <int NV>
-__device__ __forceinline__ PreprocessState resolve(const PreprocessArgs& g) {
-    PreprocessState a;
+__device__ __forceinline__ PreprocessState<NV> resolve(const PreprocessArgs<NV>& g) {
+    PreprocessState<NV> a;
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
         a.vp[v] = g.vp[v];
@@ -340,8 +342,8 @@ __device__ __forceinline__ uint32_t tiles_touched(const ViewParams& vp, uint32_t
 }
 
 template <int NV>
-__global__ __launch_bounds__(kPrimBlock) void preprocess_aaa_kernel(PreprocessArgs g) {
-    const PreprocessState a = resolve<NV>(g);
+__global__ __launch_bounds__(kPrimBlock) void preprocess_aaa_kernel(PreprocessArgs<NV> g) {
+    const PreprocessState<NV> a = resolve<NV>(g);
     __shared__ uint32_t s_sum[NV][kPrimBlock / 64];
     __shared__ uint32_t s_vis[NV][kPrimBlock / 64];
     const uint32_t i = blockIdx.x * kPrimBlock + threadIdx.x;
@@ -418,8 +420,8 @@ __global__ __launch_bounds__(kPrimBlock) void preprocess_aaa_kernel(PreprocessAr
 }
 
 template <int NV>
-__global__ __launch_bounds__(kPrimBlock) void preprocess_kernel(PreprocessArgs g) {
-    const PreprocessState a = resolve<NV>(g);
+__global__ __launch_bounds__(kPrimBlock) void preprocess_kernel(PreprocessArgs<NV> g) {
+    const PreprocessState<NV> a = resolve<NV>(g);
     extern __shared__ float s_sh[];  // per wave: kShSlots slots of the first min(3S, 45) SH floats
     __shared__ uint32_t s_sum[NV][kPrimBlock / 64];
     __shared__ uint32_t s_vis[NV][kPrimBlock / 64];
@@ -549,9 +551,17 @@ __global__ __launch_bounds__(kPrimBlock) void preprocess_kernel(PreprocessArgs g
     uint64_t rest = 0ull;
     transform(0);
     if (gather) rest = issue(__ballot(visible[0]));
-    if (NV == 2) {
-        transform(NV - 1);
-        if (gather) rest |= issue(__ballot(visible[NV - 1] && !visible[0]));
+    if constexpr (NV == 2) {
+        transform(1);
+        if (gather) rest |= issue(__ballot(visible[1] && !visible[0]));
+    } else if constexpr (NV > 2) {
+        bool seen = visible[0];  // visible in a view before v: its coefficients are on their way
+#pragma unroll
+        for (int v = 1; v < NV; ++v) {
+            transform(v);
+            if (gather) rest |= issue(__ballot(visible[v] && !seen));
+            seen |= visible[v];
+        }
     }
     // evaluates the lanes holding a slot, then frees the slots
     auto evaluate = [&]() {
@@ -629,29 +639,37 @@ __global__ __launch_bounds__(kPrimBlock) void preprocess_kernel(PreprocessArgs g
     }
 }
 
-hipError_t launch_preprocess(const ViewParams* const* vps, const ViewBuffers* vbs, int n_views,
-                             const SceneArgs& scene, uint32_t n, hipStream_t s) {
-    if (n == 0) return hipSuccess;
-    if (n_views < 1 || n_views > kMaxViewsPerPass) return hipErrorInvalidValue;
-    PreprocessArgs a;
-    for (int v = 0; v < kMaxViewsPerPass; ++v) {
-        const int k = v < n_views ? v : 0;
-        a.vp[v] = *vps[k];
-        a.vb[v] = vbs[k];
+template <int NV>
+hipError_t launch_preprocess_nv(const ViewParams* const* vps, const ViewBuffers* vbs, const SceneArgs& scene,
+                                uint32_t n, hipStream_t s) {
+    PreprocessArgs<NV> a;
+    for (int v = 0; v < NV; ++v) {
+        a.vp[v] = *vps[v];
+        a.vb[v] = vbs[v];
     }
     a.scene = scene;
     a.n = n;
     const uint32_t blocks = (n + kPrimBlock - 1) / kPrimBlock;
     // inference.cuh:86-112: the anti-aliased variant replaces transform and cull
     if (vps[0]->aaa) {
-        if (n_views == 1) preprocess_aaa_kernel<1><<<blocks, kPrimBlock, 0, s>>>(a);
-        else preprocess_aaa_kernel<2><<<blocks, kPrimBlock, 0, s>>>(a);
+        preprocess_aaa_kernel<NV><<<blocks, kPrimBlock, 0, s>>>(a);
     } else {
         const size_t lds = sizeof(float) * (size_t)(kPrimBlock / 64) * kShSlots * (size_t)min(3 * vps[0]->total_sh, 45);
-        if (n_views == 1) preprocess_kernel<1><<<blocks, kPrimBlock, lds, s>>>(a);
-        else preprocess_kernel<2><<<blocks, kPrimBlock, lds, s>>>(a);
+        preprocess_kernel<NV><<<blocks, kPrimBlock, lds, s>>>(a);
     }
     return hipGetLastError();
+}
+
+hipError_t launch_preprocess(const ViewParams* const* vps, const ViewBuffers* vbs, int n_views,
+                             const SceneArgs& scene, uint32_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    switch (n_views) {
+        case 1: return launch_preprocess_nv<1>(vps, vbs, scene, n, s);
+        case 2: return launch_preprocess_nv<2>(vps, vbs, scene, n, s);
+        case 3: return launch_preprocess_nv<3>(vps, vbs, scene, n, s);
+        case 4: return launch_preprocess_nv<4>(vps, vbs, scene, n, s);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace fhtgs

@@ -169,11 +169,12 @@ int fhtgs_render(fhtgs_handle h, const fhtgs_scene* scene, const fhtgs_view* vie
  * scenes[view_scene[v]] and uses the handle's slot view_slot[v] (NULL: slot
  * v) -- a slot keeps its buffers and instance capacity between calls, so a
  * viewer that always uses the same slots keeps its sort buffers resident.
- * Slots of one call must differ (0 <= slot < 4096). Two views of one scene
- * that agree on the preprocess parameters share one preprocess pass (each
- * view is paired with the next unpaired such view of the call, adjacent or
- * not: stereo pairs, or mono views of one scene anywhere in the batch). Each
- * view's image equals what fhtgs_render gives for it alone.
+ * Slots of one call must differ (0 <= slot < 4096). Up to four views of one
+ * scene that agree on the preprocess parameters share one preprocess pass,
+ * which reads the Gaussians once for all of them (each view is grouped with
+ * the next ungrouped such views of the call, adjacent or not: stereo pairs,
+ * or views of one scene anywhere in the batch). Each view's image equals
+ * what fhtgs_render gives for it alone.
  * fhtgs_debug_read/counts address slots. */
 int fhtgs_render_batch(fhtgs_handle h, const fhtgs_scene* scenes, int32_t n_scenes,
                        const fhtgs_view* views, const int32_t* view_scene,
