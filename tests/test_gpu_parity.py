@@ -48,6 +48,11 @@ CASES = [
     Case("cfg1_100k_800x800_k4_nofov", 100_000, 800, 800, seed=0, cube=10.0, K=4, foveation_radius_px=-1),
     # the plain GPU configuration without foveation: 1M Gaussians, 20 m cube, 1920x1080 at 60 deg, K=8
     Case("cfg2_1M_1920x1080_nofov", 1_000_000, 1920, 1080, seed=0, cube=20.0, K=8, foveation_radius_px=-1),
+    # scenes whose instance kernel takes 512 and 1024 Gaussians a workgroup (fewer than one
+    # 2048-Gaussian workgroup per CU; 0.1M above takes 256, the 1M scenes 2048)
+    Case("inst_512_per_wg_150k", 150_000, 640, 480, seed=24, cube=12.0),
+    Case("inst_1024_per_wg_300k", 300_000, 800, 600, seed=25, cube=14.0, foveation_radius_px=200,
+         blend_width_px=60),
     # more than 65535 small tiles: 32-bit sort keys (end_bit 17) through the whole pipeline
     Case("u32_keys_2048x2080", 20000, 2048, 2080, seed=23, foveation_radius_px=300, blend_width_px=100),
     Case("aaa_fov", 30000, 640, 480, seed=41, aaa=True),
