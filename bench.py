@@ -571,8 +571,8 @@ def serving_summary(args) -> dict:
     """Capacity of the serving loop: 8 headsets (stereo 2064x2208 at 100
     deg, the three levels) over the 8 resident scenes (24 M Gaussians), at
     offered loads of `--load` x 90 Hz per headset, served with continuous
-    batching on two workers (batches of one headset, and of up to two) and
-    on the plain per-request path. Capacity = the highest served stereo
+    batching on `--serve-workers` workers (batches of one headset, and of up
+    to two) and on the plain per-request path. Capacity = the highest served stereo
     frames/s among the loads whose p99 latency is within 11.1 ms (90 Hz)."""
     import torch
     from foveatedhtgscudabackend_amd import serving as V
@@ -581,7 +581,7 @@ def serving_summary(args) -> dict:
     radii = foveation_radii(args.levels, width, fov)
     loads = args.load or (1.0, 1.5, 2.0)
     duration = 1.0
-    srv = V.Server(scenes, workers=2, max_batch_headsets=2, foveation=radii, keep_images=False)
+    srv = V.Server(scenes, workers=args.serve_workers, max_batch_headsets=2, foveation=radii, keep_images=False)
     admitted = [h for h in range(8) if srv.admit(h, h, width, height)]
     sweep, warmed = [], False
     for m in loads:
